@@ -1,0 +1,443 @@
+// Merge-path ("intersect path", technique `ip`) kernels for gfx950.
+//
+// Reference behaviour: src/ip.cu:141-177 (driver), findDiagonals
+// (include/helpers.cuh:74-166) and intersectPath (ip.cu:207-245): the merge
+// path of a pair is cut into equal-length partitions, one per block, and each
+// thread merges `vt` elements of its block's partition, counting equal pairs.
+//
+// MI355X design (not a translation):
+//  * One workgroup owns one contiguous SEGMENT of the merge path (SISI) or one
+//    whole pair (MISI).  Only the segment ends need a global co-rank search;
+//    it is done by half-waves with a 32-ary search (ceil(log32 n) dependent
+//    rounds instead of log2 n).
+//  * Inside the segment the workgroup streams tiles of T = NT*VT merged
+//    elements through two circular LDS windows (T elements of A, T of B).
+//    After each tile only the consumed elements are re-fetched, with aligned
+//    128-bit global loads issued BEFORE the tile's merge and committed to LDS
+//    after it, so HBM traffic is one read of each input element and the load
+//    latency overlaps the merge.
+//  * Per tile every thread finds its merge-path split by binary search in LDS
+//    at the END of its VT-element diagonal range; its start is its
+//    neighbour's end, so each thread does one search.
+//  * Ties put the A element first, so an A element is matched against the B
+//    element that is next in line when it is merged; no partition can split a
+//    matching pair and no "count augmentation" (helpers.cuh:121-123) is needed.
+//  * Counts are reduced with wave64 shuffles, one value per workgroup; the
+//    last workgroup to finish folds the per-segment partials (no atomics on
+//    the result, no memset before a launch).
+//  * Materialisation writes each segment's matches, in order, to a staging
+//    slot; a second kernel places every slot at its prefix-sum offset.
+#include "si_device.h"
+#include "si_internal.h"
+
+namespace {
+
+using namespace si_dev;
+
+template <int NT, int VT>
+struct MergeCfg {
+    static constexpr int T = NT * VT;          // merged elements per tile
+    static constexpr uint32_t M = T - 1;       // circular window mask
+    static constexpr int CPT = (T / 4 + 4 + NT - 1) / NT;  // 16-byte chunks per thread per refill
+    static_assert((T & (T - 1)) == 0, "tile must be a power of two");
+};
+
+template <int NT, int VT>
+struct MergeSmem {
+    uint32_t wa[NT * VT];
+    uint32_t wb[NT * VT];
+    uint32_t ends[NT];
+    uint32_t wave_tot[NT / WAVE];
+    uint64_t red[NT / WAVE];
+    uint64_t seg[4];
+};
+
+// Loads of one window refill, held in registers across the tile's merge.
+template <int NT, int VT>
+struct Refill {
+    using C = MergeCfg<NT, VT>;
+    uint4 v[C::CPT];
+    uint64_t cA0, cB0;     // first chunk index of the A / B ranges
+    uint32_t nCA, nCB;     // chunk counts
+    uint64_t gA0, gA1, gB0, gB1;  // element ranges [g0, g1) to write
+
+    __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t a0, uint64_t a1, uint64_t b0,
+                                          uint64_t b1) {
+        gA0 = a0;
+        gA1 = a1;
+        gB0 = b0;
+        gB1 = b1;
+        cA0 = a0 >> 2;
+        nCA = a1 > a0 ? uint32_t(((a1 + 3) >> 2) - cA0) : 0u;
+        cB0 = b0 >> 2;
+        nCB = b1 > b0 ? uint32_t(((b1 + 3) >> 2) - cB0) : 0u;
+        const uint4* __restrict__ b4 = reinterpret_cast<const uint4*>(base);
+#pragma unroll
+        for (int j = 0; j < C::CPT; ++j) {
+            const uint32_t c = threadIdx.x + uint32_t(j) * NT;
+            if (c < nCA)
+                v[j] = b4[cA0 + c];
+            else if (c < nCA + nCB)
+                v[j] = b4[cB0 + (c - nCA)];
+        }
+    }
+
+    __device__ __forceinline__ static void put(uint32_t* w, const uint4& x, uint64_t g, uint64_t lo, uint64_t hi) {
+        if (g >= lo && g + 4 <= hi) {
+            *reinterpret_cast<uint4*>(&w[uint32_t(g) & C::M]) = x;
+        } else {
+            if (g + 0 >= lo && g + 0 < hi) w[uint32_t(g + 0) & C::M] = x.x;
+            if (g + 1 >= lo && g + 1 < hi) w[uint32_t(g + 1) & C::M] = x.y;
+            if (g + 2 >= lo && g + 2 < hi) w[uint32_t(g + 2) & C::M] = x.z;
+            if (g + 3 >= lo && g + 3 < hi) w[uint32_t(g + 3) & C::M] = x.w;
+        }
+    }
+
+    __device__ __forceinline__ void commit(MergeSmem<NT, VT>& sm) const {
+#pragma unroll
+        for (int j = 0; j < C::CPT; ++j) {
+            const uint32_t c = threadIdx.x + uint32_t(j) * NT;
+            if (c < nCA)
+                put(sm.wa, v[j], (cA0 + c) * 4, gA0, gA1);
+            else if (c < nCA + nCB)
+                put(sm.wb, v[j], (cB0 + (c - nCA)) * 4, gB0, gB1);
+        }
+    }
+};
+
+// Streams the merge path from (ia, ib) to (iaEnd, ibEnd) of sets A (global
+// element offset oA, size nA) and B (oB, nB) through the LDS windows.
+// Returns this thread's match count.  With MAT, matches are written in order
+// to stage[0 ..) (the segment's staging slot).
+template <int NT, int VT, bool MAT>
+__device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
+                                  uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
+                                  MergeSmem<NT, VT>& sm, uint32_t* __restrict__ stage) {
+    using C = MergeCfg<NT, VT>;
+    constexpr uint32_t T = C::T;
+    const uint32_t tid = threadIdx.x;
+    uint64_t myCount = 0;
+    uint64_t written = 0;  // MAT: matches already staged by this workgroup
+
+    uint64_t d = ia + ib;
+    const uint64_t dEnd = iaEnd + ibEnd;
+    if (d >= dEnd) return 0;
+
+    // initial fill: A[ia, ia+T) and B[ib, ib+T), clipped to the sets
+    {
+        Refill<NT, VT> r;
+        const uint64_t a1 = (ia + T < nA ? ia + T : nA);
+        r.issue(base, oA + ia, oA + a1, 0, 0);
+        r.commit(sm);
+        const uint64_t b1 = (ib + T < nB ? ib + T : nB);
+        r.issue(base, 0, 0, oB + ib, oB + b1);
+        r.commit(sm);
+    }
+    __syncthreads();
+
+    while (d < dEnd) {
+        const uint32_t Tc = uint32_t((dEnd - d) < T ? (dEnd - d) : T);
+        const uint32_t nAw = uint32_t((nA - ia) < T ? (nA - ia) : T);
+        const uint32_t nBw = uint32_t((nB - ib) < T ? (nB - ib) : T);
+        const uint32_t gA = uint32_t(oA + ia);  // slot base (mod T) of the windows
+        const uint32_t gB = uint32_t(oB + ib);
+        auto WA = [&](uint32_t i) { return sm.wa[(gA + i) & C::M]; };
+        auto WB = [&](uint32_t i) { return sm.wb[(gB + i) & C::M]; };
+
+        // 1. split at the end of this thread's diagonal range
+        const uint32_t dd = ((tid + 1) * VT < Tc) ? (tid + 1) * VT : Tc;
+        {
+            uint32_t lo = dd > nBw ? dd - nBw : 0u;
+            uint32_t hi = dd < nAw ? dd : nAw;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (WA(mid) <= WB(dd - 1 - mid))
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            sm.ends[tid] = lo;
+        }
+        __syncthreads();
+        const uint32_t ds = (tid * VT < Tc) ? tid * VT : Tc;
+        const uint32_t x0 = tid ? sm.ends[tid - 1] : 0u;
+        const uint32_t x1 = sm.ends[tid];
+        const uint32_t y0 = ds - x0;
+        const uint32_t y1 = dd - x1;
+        const uint32_t ca = sm.ends[NT - 1];
+        const uint32_t cb = Tc - ca;
+
+        // 2. fetch the elements that replace the consumed ones (registers)
+        Refill<NT, VT> r;
+        {
+            const uint64_t a0 = ia + T < nA ? ia + T : nA;
+            const uint64_t a1 = ia + T + ca < nA ? ia + T + ca : nA;
+            const uint64_t b0 = ib + T < nB ? ib + T : nB;
+            const uint64_t b1 = ib + T + cb < nB ? ib + T + cb : nB;
+            r.issue(base, oA + a0, oA + a1, oB + b0, oB + b1);
+        }
+
+        // 3. merge this thread's VT steps; A is taken on ties and matched
+        //    against the current B head (which may lie past y1).
+        uint32_t x = x0, y = y0;
+        uint32_t av = x < x1 ? WA(x) : 0u;
+        uint32_t bv = y < nBw ? WB(y) : 0u;
+        const uint32_t steps = (x1 - x0) + (y1 - y0);
+        uint32_t cnt = 0;
+        uint32_t mval[MAT ? VT : 1];
+        uint32_t mbits = 0;
+#pragma unroll
+        for (int k = 0; k < VT; ++k) {
+            if (uint32_t(k) < steps) {
+                const bool takeA = (x < x1) && (y >= y1 || av <= bv);
+                const bool hit = takeA && (y < nBw) && (av == bv);
+                cnt += hit;
+                if (MAT) {
+                    mval[k] = av;
+                    mbits |= uint32_t(hit) << k;
+                }
+                if (takeA) {
+                    ++x;
+                    if (x < x1) av = WA(x);
+                } else {
+                    ++y;
+                    if (y < nBw) bv = WB(y);
+                }
+            }
+        }
+        myCount += cnt;
+
+        uint32_t excl = 0, wtot = 0;
+        if (MAT) {
+            const uint32_t incl = wave_inclusive_scan_u32(cnt);
+            excl = incl - cnt;
+            if (lane_id() == WAVE - 1) sm.wave_tot[tid / WAVE] = incl;
+        }
+        __syncthreads();  // all window reads of this tile are done
+
+        // 4. commit the refill; with MAT, stage this tile's matches
+        r.commit(sm);
+        if (MAT) {
+            const uint32_t w = tid / WAVE;
+            uint32_t before = 0;
+#pragma unroll
+            for (int i = 0; i < NT / WAVE; ++i) {
+                const uint32_t t = sm.wave_tot[i];
+                if (i < int(w)) before += t;
+                wtot += t;
+            }
+            uint64_t pos = written + before + excl;
+#pragma unroll
+            for (int k = 0; k < VT; ++k) {
+                if (mbits & (1u << k)) stage[pos++] = mval[k];
+            }
+            written += wtot;
+        }
+        ia += ca;
+        ib += cb;
+        d += Tc;
+        __syncthreads();
+    }
+    return myCount;
+}
+
+// ---------------------------------------------------------------- SISI ----
+// One workgroup per merge-path segment of the pair (A, B).
+//   partials[s]     per-segment count (MAT) / partial sum (count mode)
+//   done            per-launch ticket counter, left at zero
+//   out             final uint64 count (count mode)
+template <int NT, int VT, bool MAT>
+__global__ __launch_bounds__(NT) void k_sisi_merge(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
+                                                   uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
+                                                   uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
+                                                   uint64_t* __restrict__ out, uint32_t* __restrict__ stage,
+                                                   uint64_t stage_cap) {
+    __shared__ MergeSmem<NT, VT> sm;
+    const uint32_t s = blockIdx.x;
+    const uint64_t n = nA + nB;
+    const uint64_t dBeg = uint64_t(s) * seg_len;
+    const uint64_t dEnd = (dBeg + seg_len < n) ? dBeg + seg_len : n;
+
+    if (threadIdx.x < WAVE) {
+        const uint64_t dd = (lane_id() < 32) ? dBeg : dEnd;
+        const uint64_t c = corank_halfwave(base + oA, nA, base + oB, nB, dd);
+        if (lane_id() == 0) sm.seg[0] = c;
+        if (lane_id() == 32) sm.seg[1] = c;
+    }
+    __syncthreads();
+    const uint64_t ia = sm.seg[0], iaE = sm.seg[1];
+    const uint64_t ib = dBeg - ia, ibE = dEnd - iaE;
+
+    uint64_t c = merge_segment<NT, VT, MAT>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm,
+                                            MAT ? stage + uint64_t(s) * stage_cap : nullptr);
+    const uint64_t total = block_sum_u64<NT>(c, sm.red);
+    if (threadIdx.x == 0) partials[s] = total;
+    if (MAT) return;
+
+    // last workgroup folds the partials
+    __shared__ uint32_t isLast;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t ticket = atomicAdd(done, 1u);
+        isLast = (ticket == nseg - 1);
+    }
+    __syncthreads();
+    if (!isLast) return;
+    __threadfence();
+    uint64_t acc = 0;
+    for (uint32_t i = threadIdx.x; i < nseg; i += NT) acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc = block_sum_u64<NT>(acc, sm.red);
+    if (threadIdx.x == 0) {
+        *out = acc;
+        *done = 0;
+    }
+}
+
+// Places each segment's staged matches at its prefix offset.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ counts, uint32_t nseg,
+                                               const uint32_t* __restrict__ stage, uint64_t stage_cap,
+                                               uint32_t* __restrict__ out, uint64_t* __restrict__ total) {
+    __shared__ uint64_t red[NT / WAVE];
+    const uint32_t s = blockIdx.x;
+    uint64_t acc = 0;
+    for (uint32_t i = threadIdx.x; i < s; i += NT) acc += counts[i];
+    const uint64_t before = block_sum_u64<NT>(acc, red);
+    const uint64_t cnt = counts[s];
+    const uint32_t* src = stage + uint64_t(s) * stage_cap;
+    uint32_t* dst = out + before;
+    for (uint64_t i = threadIdx.x; i < cnt; i += NT) dst[i] = src[i];
+    if (s == nseg - 1 && threadIdx.x == 0) *total = before + cnt;
+}
+
+// ---------------------------------------------------------------- MISI ----
+// One 64-thread workgroup per pair, grid-stride over the pairs of a run.
+template <int VT, bool SELF>
+__global__ __launch_bounds__(WAVE) void k_run_merge(const uint32_t* __restrict__ base,
+                                                    const uint64_t* __restrict__ offsets, uint32_t aStart,
+                                                    uint32_t aLen, uint32_t aId, uint32_t bStart, uint32_t bLen,
+                                                    uint32_t bId, uint32_t n, uint64_t npairs,
+                                                    uint32_t* __restrict__ counts) {
+    __shared__ MergeSmem<WAVE, VT> sm;
+    for (uint64_t p = blockIdx.x; p < npairs; p += gridDim.x) {
+        uint32_t li, lj;
+        if (SELF) {
+            tri_decode(p, aLen, li, lj);
+        } else {
+            li = uint32_t(p / bLen);
+            lj = uint32_t(p % bLen);
+        }
+        const uint32_t a = aStart + li, b = bStart + lj;
+        const uint64_t oA = offsets[a], nA = offsets[a + 1] - oA;
+        const uint64_t oB = offsets[b], nB = offsets[b + 1] - oB;
+        uint64_t c = 0;
+        if (nA && nB) c = merge_segment<WAVE, VT, false>(base, oA, nA, oB, nB, 0, 0, nA, nB, sm, nullptr);
+        c = wave_sum_u64(c);
+        if (threadIdx.x == 0) {
+            const uint64_t ia = uint64_t(a) - uint64_t(aId) * n, ib = uint64_t(b) - uint64_t(bId) * n;
+            const uint64_t idx = SELF ? tri_index(n, ia, ib) : ia * n + ib;
+            counts[idx] = uint32_t(c);
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int SISI_NT = 256;
+constexpr int SISI_VT = 16;
+constexpr int RUN_VT = 8;
+
+}  // namespace
+
+namespace si_impl {
+
+static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, uint32_t& nseg, uint64_t& seg_len) {
+    constexpr uint64_t T = uint64_t(SISI_NT) * SISI_VT;
+    uint64_t maxSeg = uint64_t(ctx->num_cus) * 4;
+    if (p && p->blocks) maxSeg = p->blocks;
+    uint64_t want = (n + T - 1) / T;
+    if (want < 1) want = 1;
+    if (want > maxSeg) want = maxSeg;
+    seg_len = (n + want - 1) / want;
+    if (seg_len < 1) seg_len = 1;
+    nseg = uint32_t((n + seg_len - 1) / seg_len);
+    if (nseg == 0) nseg = 1;
+}
+
+int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out) {
+    si_context* ctx = ds->ctx;
+    const uint64_t oA = ds->offsets[a], nA = ds->sizes[a];
+    const uint64_t oB = ds->offsets[b], nB = ds->sizes[b];
+    uint32_t nseg;
+    uint64_t seg_len;
+    sisi_geometry(ctx, nA + nB, p, nseg, seg_len);
+    void* ws = nullptr;
+    SI_TRY(small_workspace(ctx, 256 + uint64_t(nseg) * 8, &ws));
+    uint32_t* done = static_cast<uint32_t*>(ws);
+    uint64_t* partials = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256);
+    if (nA == 0 || nB == 0) {
+        SI_HIP(hipMemsetAsync(d_out, 0, 8, ctx->stream));
+        return SI_OK;
+    }
+    KernelScope ks(ctx, "ip_sisi_count");
+    hipLaunchKernelGGL((k_sisi_merge<SISI_NT, SISI_VT, false>), dim3(nseg), dim3(SISI_NT), 0, ctx->stream,
+                       ds->d_elems, oA, nA, oB, nB, seg_len, nseg, partials, done, d_out, nullptr, uint64_t(0));
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out, uint64_t* d_count) {
+    si_context* ctx = ds->ctx;
+    const uint64_t oA = ds->offsets[a], nA = ds->sizes[a];
+    const uint64_t oB = ds->offsets[b], nB = ds->sizes[b];
+    if (nA == 0 || nB == 0) {
+        SI_HIP(hipMemsetAsync(d_count, 0, 8, ctx->stream));
+        return SI_OK;
+    }
+    uint32_t nseg;
+    uint64_t seg_len;
+    sisi_geometry(ctx, nA + nB, nullptr, nseg, seg_len);
+    const uint64_t cap = ((seg_len / 2 + 2) + 63) & ~uint64_t(63);
+    void* ws = nullptr;
+    SI_TRY(small_workspace(ctx, 256 + uint64_t(nseg) * 8, &ws));
+    uint32_t* done = static_cast<uint32_t*>(ws);
+    uint64_t* partials = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256);
+    SI_TRY(ctx->ws_stage.ensure(cap * nseg * sizeof(uint32_t)));
+    uint32_t* stage = ctx->ws_stage.as<uint32_t>();
+    {
+        KernelScope ks(ctx, "ip_sisi_materialize");
+        hipLaunchKernelGGL((k_sisi_merge<SISI_NT, SISI_VT, true>), dim3(nseg), dim3(SISI_NT), 0, ctx->stream,
+                           ds->d_elems, oA, nA, oB, nB, seg_len, nseg, partials, done, d_count, stage, cap);
+        SI_HIP(hipGetLastError());
+    }
+    {
+        KernelScope ks(ctx, "ip_sisi_compact");
+        hipLaunchKernelGGL((k_compact<256>), dim3(nseg), dim3(256), 0, ctx->stream, partials, nseg, stage, cap, d_out,
+                           d_count);
+        SI_HIP(hipGetLastError());
+    }
+    return SI_OK;
+}
+
+int ip_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts) {
+    si_context* ctx = ds->ctx;
+    const bool self = (A.id == B.id);
+    const uint64_t aLen = A.end - A.start, bLen = B.end - B.start;
+    const uint64_t npairs = self ? (aLen * (aLen - (aLen > 0)) / 2) : aLen * bLen;
+    if (npairs == 0) return SI_OK;
+    uint64_t grid = uint64_t(ctx->num_cus) * 32;
+    if (p && p->blocks) grid = p->blocks;
+    if (grid > npairs) grid = npairs;
+    KernelScope ks(ctx, "ip_run");
+    if (self)
+        hipLaunchKernelGGL((k_run_merge<RUN_VT, true>), dim3(uint32_t(grid)), dim3(WAVE), 0, ctx->stream, ds->d_elems,
+                           ds->d_offsets, A.start, uint32_t(aLen), A.id, B.start, uint32_t(bLen), B.id, n, npairs,
+                           d_counts);
+    else
+        hipLaunchKernelGGL((k_run_merge<RUN_VT, false>), dim3(uint32_t(grid)), dim3(WAVE), 0, ctx->stream, ds->d_elems,
+                           ds->d_offsets, A.start, uint32_t(aLen), A.id, B.start, uint32_t(bLen), B.id, n, npairs,
+                           d_counts);
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+}  // namespace si_impl

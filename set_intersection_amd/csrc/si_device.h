@@ -1,0 +1,136 @@
+// Device-side building blocks shared by the gfx950 kernels: wave64 reductions
+// and scans, merge-path co-rank searches, and pair-index decoding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace si_dev {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// --------------------------------------------------------------- reductions
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Inclusive prefix sum across the wave.
+__device__ __forceinline__ uint32_t wave_inclusive_scan_u32(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        uint32_t t = __shfl_up(v, off, WAVE);
+        if (lane >= uint32_t(off)) v += t;
+    }
+    return v;
+}
+
+// Sum over a workgroup of NT threads; `scratch` holds NT/64 uint64 slots.
+// Every thread receives the total.  Contains two barriers.
+template <int NT>
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* scratch) {
+    v = wave_sum_u64(v);
+    const int w = threadIdx.x / WAVE;
+    if (NT == WAVE) return v;
+    if (lane_id() == 0) scratch[w] = v;
+    __syncthreads();
+    uint64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < NT / WAVE; ++i) t += scratch[i];
+    __syncthreads();
+    return t;
+}
+
+// ------------------------------------------------------------- co-rank search
+// Merge order used everywhere: A[x] precedes B[y] iff A[x] <= B[y] (ties put
+// the A element first).  corank(d) = number of A elements among the first d
+// merged elements = the smallest i in [max(0,d-nB), min(d,nA)] with
+// A[i] > B[d-1-i] (or the upper bound if there is none).
+//
+// With this order the matching partner of A[x] is always the B element that
+// is next in line when A[x] is merged, so an A element taken inside any
+// partition of the merge path only ever needs the B "head" that follows the
+// partition's B range.  No tie can be split between partitions.
+
+// Half-wave (32 lanes) cooperative co-rank search in global memory: 32-ary
+// search, ceil(log32(n)) dependent rounds.  `half` selects lanes 0-31 or
+// 32-63; both halves may search different diagonals concurrently.  All lanes
+// of the half must call it with the same arguments.
+__device__ __forceinline__ uint64_t corank_halfwave(const uint32_t* __restrict__ A, uint64_t nA,
+                                                    const uint32_t* __restrict__ B, uint64_t nB, uint64_t d) {
+    const uint32_t lane = lane_id() & 31u;
+    const uint32_t half = lane_id() >> 5;
+    uint64_t lo = d > nB ? d - nB : 0;
+    uint64_t hi = d < nA ? d : nA;
+    while (lo < hi) {
+        const uint64_t len = hi - lo;
+        const uint64_t step = (len + 31) >> 5;
+        const uint64_t p = lo + uint64_t(lane) * step;
+        bool pred = false;
+        if (p < hi) pred = A[p] > B[d - 1 - p];
+        const uint64_t m64 = __ballot(pred);
+        const uint32_t m = uint32_t(m64 >> (32 * half));
+        if (m) {
+            const uint32_t f = __builtin_ctz(m);
+            const uint64_t pf = lo + uint64_t(f) * step;
+            const uint64_t newlo = f ? lo + uint64_t(f - 1) * step + 1 : lo;
+            lo = newlo;
+            hi = pf;
+        } else {
+            const uint64_t last = (hi - 1 - lo) / step;  // last probed lane (<= 31)
+            lo = lo + last * step + 1;
+        }
+    }
+    return lo;
+}
+
+// Single-thread binary co-rank search (used for short ranges).
+__device__ __forceinline__ uint64_t corank_serial(const uint32_t* __restrict__ A, uint64_t nA,
+                                                  const uint32_t* __restrict__ B, uint64_t nB, uint64_t d) {
+    uint64_t lo = d > nB ? d - nB : 0;
+    uint64_t hi = d < nA ? d : nA;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (A[mid] <= B[d - 1 - mid])
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------ pair indexing
+// triangular_index of include/util.hpp:68-70 in 64-bit arithmetic.
+__device__ __forceinline__ uint64_t tri_index(uint64_t n, uint64_t i, uint64_t j) {
+    return (n * (n - 1) / 2) - (n - i) * ((n - i) - 1) / 2 + j - i - 1;
+}
+
+// Inverse of the row-major strict-upper-triangle enumeration of an L x L
+// tile: p -> (i, j), 0 <= i < j < L.
+__device__ __forceinline__ void tri_decode(uint64_t p, uint64_t L, uint32_t& i, uint32_t& j) {
+    // row i starts at r(i) = i*L - i*(i+1)/2
+    const double Ld = double(L);
+    double disc = (2.0 * Ld - 1.0) * (2.0 * Ld - 1.0) - 8.0 * double(p);
+    if (disc < 0) disc = 0;
+    int64_t ii = int64_t(((2.0 * Ld - 1.0) - sqrt(disc)) * 0.5);
+    if (ii < 0) ii = 0;
+    if (ii > int64_t(L) - 2) ii = int64_t(L) - 2;
+    auto rowStart = [&](int64_t r) { return uint64_t(r) * L - uint64_t(r) * uint64_t(r + 1) / 2; };
+    while (ii > 0 && rowStart(ii) > p) --ii;
+    while (ii + 1 <= int64_t(L) - 2 && rowStart(ii + 1) <= p) ++ii;
+    i = uint32_t(ii);
+    j = uint32_t(p - rowStart(ii) + uint64_t(ii) + 1);
+}
+
+}  // namespace si_dev

@@ -1,0 +1,117 @@
+// Internal declarations of libsi.so (not installed; the public ABI is
+// include/set_intersection.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "set_intersection.h"
+
+namespace si_impl {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+
+#define SI_HIP(call)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return ::si_impl::fail(SI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define SI_TRY(expr)          \
+    do {                      \
+        int s_ = (expr);      \
+        if (s_ != SI_OK) return s_; \
+    } while (0)
+
+// Device memory owned by the library, grown on demand.
+struct DevMem {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+    int ensure(uint64_t want);  // grows (never shrinks); contents are not kept
+    void release();
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace si_impl
+
+struct si_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop{};
+    int num_cus = 0;
+    // DeviceTimer events
+    std::vector<hipEvent_t> events;
+    // per-kernel timing
+    bool kernel_timing = false;
+    struct Pending {
+        std::string name;
+        hipEvent_t start, end;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+    struct KStat {
+        double ms = 0.0;
+        uint64_t launches = 0;
+    };
+    std::map<std::string, KStat> kstats;
+    // scratch workspaces (reused across calls on this context)
+    si_impl::DevMem ws_small;   // counters, per-segment counts, reduction partials
+    si_impl::DevMem ws_stage;   // materialisation staging
+    si_impl::DevMem ws_aux;     // technique-specific (bitmaps, hash bins, pair lists)
+    si_impl::DevMem ws_aux2;
+    bool ws_small_zeroed = false;
+};
+
+struct si_buffer {
+    si_context* ctx = nullptr;
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct si_dataset {
+    si_context* ctx = nullptr;
+    uint32_t k = 0;
+    uint32_t universe = 0;
+    uint64_t total = 0;
+    uint32_t* d_elems = nullptr;    // total elements (+ 64 padding elements)
+    uint32_t* d_sizes = nullptr;    // k
+    uint64_t* d_offsets = nullptr;  // k + 1
+    std::vector<uint32_t> sizes;    // host copies for planning
+    std::vector<uint64_t> offsets;  // k + 1
+    uint32_t max_size = 0;
+    uint32_t max_elem = 0;          // largest element of the collection
+};
+
+namespace si_impl {
+
+// Kernel timing bracket: records events around a launch when enabled.
+struct KernelScope {
+    si_context* ctx;
+    const char* name;
+    hipEvent_t start = nullptr, end = nullptr;
+    KernelScope(si_context* c, const char* n);
+    ~KernelScope();
+};
+
+int check_pair(si_dataset* ds, uint32_t a, uint32_t b);
+
+// Zero-initialised per-context counters region of `bytes` bytes.  Kernels
+// that use it leave it zeroed again when they finish.
+int small_workspace(si_context* ctx, uint64_t bytes, void** out);
+
+// ----------------------------------------------------- technique launchers --
+// SISI merge path (k_merge.hip)
+int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out);
+int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out, uint64_t* d_count);
+// MISI runs
+int ip_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
+
+}  // namespace si_impl

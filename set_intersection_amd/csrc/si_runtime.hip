@@ -1,0 +1,409 @@
+// libsi.so runtime: contexts, device memory, datasets, event timing, and the
+// C-ABI entry points that dispatch to the technique launchers.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+#include "si_device.h"
+#include "si_internal.h"
+
+namespace si_impl {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int DevMem::ensure(uint64_t want) {
+    if (want <= bytes && ptr) return SI_OK;
+    release();
+    if (want == 0) want = 256;
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        return fail(SI_ERR_NOMEM, "hipMalloc(" + std::to_string(want) + " B) failed: " + hipGetErrorString(e));
+    }
+    bytes = want;
+    return SI_OK;
+}
+
+void DevMem::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+static hipEvent_t take_event(si_context* ctx) {
+    if (!ctx->free_events.empty()) {
+        hipEvent_t e = ctx->free_events.back();
+        ctx->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Folds finished kernel timings into the per-name sums.
+static void drain_pending(si_context* ctx) {
+    for (auto& p : ctx->pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(p.end) == hipSuccess && hipEventElapsedTime(&ms, p.start, p.end) == hipSuccess) {
+            auto& s = ctx->kstats[p.name];
+            s.ms += ms;
+            s.launches += 1;
+        }
+        ctx->free_events.push_back(p.start);
+        ctx->free_events.push_back(p.end);
+    }
+    ctx->pending.clear();
+}
+
+KernelScope::KernelScope(si_context* c, const char* n) : ctx(c), name(n) {
+    if (!ctx->kernel_timing) return;
+    start = take_event(ctx);
+    end = take_event(ctx);
+    if (start) (void)hipEventRecord(start, ctx->stream);
+}
+
+KernelScope::~KernelScope() {
+    if (!ctx->kernel_timing || !start || !end) return;
+    (void)hipEventRecord(end, ctx->stream);
+    ctx->pending.push_back({name, start, end});
+    if (ctx->pending.size() > 4096) drain_pending(ctx);
+}
+
+int check_pair(si_dataset* ds, uint32_t a, uint32_t b) {
+    if (!ds) return fail(SI_ERR_ARG, "null dataset");
+    if (a >= ds->k || b >= ds->k)
+        return fail(SI_ERR_ARG, "set index out of range (k=" + std::to_string(ds->k) + ")");
+    return SI_OK;
+}
+
+int small_workspace(si_context* ctx, uint64_t bytes, void** out) {
+    if (bytes > ctx->ws_small.bytes || !ctx->ws_small.ptr) {
+        uint64_t want = std::max<uint64_t>(bytes, 1 << 16);
+        SI_TRY(ctx->ws_small.ensure(want));
+        ctx->ws_small_zeroed = false;
+    }
+    if (!ctx->ws_small_zeroed) {
+        SI_HIP(hipMemsetAsync(ctx->ws_small.ptr, 0, ctx->ws_small.bytes, ctx->stream));
+        ctx->ws_small_zeroed = true;
+    }
+    *out = ctx->ws_small.ptr;
+    return SI_OK;
+}
+
+}  // namespace si_impl
+
+using namespace si_impl;
+
+// ------------------------------------------------------------ validation --
+namespace {
+
+__global__ void k_offsets_scan(const uint32_t* __restrict__ sizes, uint32_t k, uint64_t* __restrict__ offsets) {
+    // single workgroup, chunked exclusive scan with 64-bit carry
+    __shared__ uint64_t carry;
+    __shared__ uint64_t wsum[16];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint64_t base = 0; base < k; base += blockDim.x) {
+        const uint64_t i = base + threadIdx.x;
+        const uint32_t v = i < k ? sizes[i] : 0u;
+        uint64_t x = v;
+        // wave inclusive scan in 64 bits
+        for (int off = 1; off < 64; off <<= 1) {
+            uint64_t t = __shfl_up(x, off, 64);
+            if ((threadIdx.x & 63) >= uint32_t(off)) x += t;
+        }
+        const int w = threadIdx.x / 64;
+        if ((threadIdx.x & 63) == 63) wsum[w] = x;
+        __syncthreads();
+        uint64_t before = carry;
+        for (int j = 0; j < w; ++j) before += wsum[j];
+        if (i < k) offsets[i] = before + x - v;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) carry = before + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) offsets[k] = carry;
+}
+
+__global__ void k_check_sorted(const uint32_t* __restrict__ elems, const uint64_t* __restrict__ offsets, uint32_t k,
+                               uint64_t total, uint32_t* __restrict__ bad) {
+    // element i is bad if it is not greater than its predecessor in the same set
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x + 1; i < total; i += stride) {
+        if (elems[i] <= elems[i - 1]) {
+            // is i the first element of a set?  binary search offsets
+            uint32_t lo = 0, hi = k;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (offsets[mid] <= i) lo = mid + 1; else hi = mid;
+            }
+            // lo-1 is the set containing i; i is its first element iff offsets[lo-1] == i
+            if (lo == 0 || offsets[lo - 1] != i) atomicAdd(bad, 1u);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* si_last_error(void) { return g_last_error.c_str(); }
+
+int si_open(int device, si_context** out) {
+    if (!out) return fail(SI_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    SI_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SI_ERR_ARG, "device " + std::to_string(device) + " not present");
+    SI_HIP(hipSetDevice(device));
+    auto* ctx = new si_context();
+    ctx->device = device;
+    hipError_t e = hipGetDeviceProperties(&ctx->prop, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(SI_ERR_HIP, std::string("si_open: ") + hipGetErrorString(e));
+    }
+    ctx->num_cus = ctx->prop.multiProcessorCount;
+    *out = ctx;
+    return SI_OK;
+}
+
+int si_close(si_context* ctx) {
+    if (!ctx) return SI_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    drain_pending(ctx);
+    for (auto e : ctx->events) (void)hipEventDestroy(e);
+    for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    ctx->ws_small.release();
+    ctx->ws_stage.release();
+    ctx->ws_aux.release();
+    ctx->ws_aux2.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SI_OK;
+}
+
+int si_device_info_get(si_context* ctx, si_device_info* out) {
+    if (!ctx || !out) return fail(SI_ERR_ARG, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    std::strncpy(out->name, ctx->prop.gcnArchName, sizeof(out->name) - 1);
+    out->multiprocessor_count = ctx->prop.multiProcessorCount;
+    out->max_threads_per_block = ctx->prop.maxThreadsPerBlock;
+    out->warp_size = ctx->prop.warpSize;
+    out->lds_per_block = int32_t(ctx->prop.sharedMemPerBlock);
+    out->total_global_mem = ctx->prop.totalGlobalMem;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) out->free_global_mem = fr;
+    out->clock_khz = ctx->prop.clockRate;
+    out->memory_clock_khz = ctx->prop.memoryClockRate;
+    out->memory_bus_width = ctx->prop.memoryBusWidth;
+    return SI_OK;
+}
+
+int si_sync(si_context* ctx) {
+    if (!ctx) return fail(SI_ERR_ARG, "null context");
+    SI_HIP(hipStreamSynchronize(ctx->stream));
+    return SI_OK;
+}
+
+// ---------------------------------------------------------------- events --
+int si_event_record(si_context* ctx, int32_t* id) {
+    if (!ctx || !id) return fail(SI_ERR_ARG, "null argument");
+    hipEvent_t e = nullptr;
+    SI_HIP(hipEventCreate(&e));
+    SI_HIP(hipEventRecord(e, ctx->stream));
+    ctx->events.push_back(e);
+    *id = int32_t(ctx->events.size() - 1);
+    return SI_OK;
+}
+
+int si_event_elapsed(si_context* ctx, int32_t a, int32_t b, float* ms) {
+    if (!ctx || !ms) return fail(SI_ERR_ARG, "null argument");
+    if (a < 0 || b < 0 || size_t(a) >= ctx->events.size() || size_t(b) >= ctx->events.size())
+        return fail(SI_ERR_ARG, "bad event id");
+    SI_HIP(hipEventSynchronize(ctx->events[b]));
+    SI_HIP(hipEventElapsedTime(ms, ctx->events[a], ctx->events[b]));
+    return SI_OK;
+}
+
+int si_event_reset(si_context* ctx) {
+    if (!ctx) return fail(SI_ERR_ARG, "null context");
+    SI_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto e : ctx->events) (void)hipEventDestroy(e);
+    ctx->events.clear();
+    return SI_OK;
+}
+
+int si_kernel_timing(si_context* ctx, int32_t enable) {
+    if (!ctx) return fail(SI_ERR_ARG, "null context");
+    ctx->kernel_timing = enable != 0;
+    return SI_OK;
+}
+
+int si_kernel_stats(si_context* ctx, const char* name, double* total_ms, uint64_t* launches) {
+    if (!ctx || !name) return fail(SI_ERR_ARG, "null argument");
+    drain_pending(ctx);
+    auto it = ctx->kstats.find(name);
+    if (total_ms) *total_ms = it == ctx->kstats.end() ? 0.0 : it->second.ms;
+    if (launches) *launches = it == ctx->kstats.end() ? 0 : it->second.launches;
+    return SI_OK;
+}
+
+int si_kernel_stats_reset(si_context* ctx) {
+    if (!ctx) return fail(SI_ERR_ARG, "null context");
+    drain_pending(ctx);
+    ctx->kstats.clear();
+    return SI_OK;
+}
+
+// ---------------------------------------------------------------- buffers --
+int si_buffer_alloc(si_context* ctx, uint64_t bytes, si_buffer** out) {
+    if (!ctx || !out) return fail(SI_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto* b = new si_buffer();
+    b->ctx = ctx;
+    b->bytes = bytes;
+    hipError_t e = hipMalloc(&b->ptr, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(SI_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+    }
+    *out = b;
+    return SI_OK;
+}
+
+int si_buffer_free(si_buffer* b) {
+    if (!b) return SI_OK;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    (void)hipFree(b->ptr);
+    delete b;
+    return SI_OK;
+}
+
+int si_buffer_device_ptr(si_buffer* b, void** p) {
+    if (!b || !p) return fail(SI_ERR_ARG, "null argument");
+    *p = b->ptr;
+    return SI_OK;
+}
+
+int si_buffer_download(si_buffer* b, uint64_t off, void* host, uint64_t bytes) {
+    if (!b || (!host && bytes)) return fail(SI_ERR_ARG, "null argument");
+    if (off + bytes > b->bytes) return fail(SI_ERR_RANGE, "download beyond buffer end");
+    if (!bytes) return SI_OK;
+    SI_HIP(hipMemcpyAsync(host, static_cast<char*>(b->ptr) + off, bytes, hipMemcpyDeviceToHost, b->ctx->stream));
+    SI_HIP(hipStreamSynchronize(b->ctx->stream));
+    return SI_OK;
+}
+
+int si_buffer_upload(si_buffer* b, uint64_t off, const void* host, uint64_t bytes) {
+    if (!b || (!host && bytes)) return fail(SI_ERR_ARG, "null argument");
+    if (off + bytes > b->bytes) return fail(SI_ERR_RANGE, "upload beyond buffer end");
+    if (!bytes) return SI_OK;
+    SI_HIP(hipMemcpyAsync(static_cast<char*>(b->ptr) + off, host, bytes, hipMemcpyHostToDevice, b->ctx->stream));
+    SI_HIP(hipStreamSynchronize(b->ctx->stream));
+    return SI_OK;
+}
+
+int si_buffer_zero(si_buffer* b) {
+    if (!b) return fail(SI_ERR_ARG, "null buffer");
+    SI_HIP(hipMemsetAsync(b->ptr, 0, b->bytes, b->ctx->stream));
+    return SI_OK;
+}
+
+// ---------------------------------------------------------------- dataset --
+int si_dataset_upload(si_context* ctx, uint32_t k, uint32_t universe, uint64_t total, const uint32_t* sizes,
+                      const uint32_t* elements, int32_t validate, si_dataset** out) {
+    if (!ctx || !out || (k && !sizes) || (total && !elements)) return fail(SI_ERR_ARG, "null argument");
+    *out = nullptr;
+    uint64_t sum = 0;
+    uint32_t mx = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        sum += sizes[i];
+        mx = std::max(mx, sizes[i]);
+    }
+    if (sum != total)
+        return fail(SI_ERR_ARG, "sum of sizes (" + std::to_string(sum) + ") != total (" + std::to_string(total) + ")");
+    auto* ds = new si_dataset();
+    ds->ctx = ctx;
+    ds->k = k;
+    ds->universe = universe;
+    ds->total = total;
+    ds->max_size = mx;
+    ds->sizes.assign(sizes, sizes + k);
+    ds->offsets.assign(size_t(k) + 1, 0);
+    for (uint32_t i = 0; i < k; ++i) ds->offsets[i + 1] = ds->offsets[i] + sizes[i];
+    auto cleanup = [&](int st, const std::string& m) {
+        si_dataset_free(ds);
+        return fail(st, m);
+    };
+    // 64 padding elements: 16-byte chunk loads may run up to 3 elements past the end
+    const uint64_t elemBytes = (total + 64) * sizeof(uint32_t);
+    if (hipMalloc(&ds->d_elems, elemBytes) != hipSuccess) return cleanup(SI_ERR_NOMEM, "hipMalloc(elements) failed");
+    if (hipMalloc(&ds->d_sizes, (uint64_t(k) + 1) * 4) != hipSuccess) return cleanup(SI_ERR_NOMEM, "hipMalloc(sizes) failed");
+    if (hipMalloc(&ds->d_offsets, (uint64_t(k) + 1) * 8) != hipSuccess)
+        return cleanup(SI_ERR_NOMEM, "hipMalloc(offsets) failed");
+    hipError_t e = hipMemsetAsync(ds->d_elems + total, 0, 64 * 4, ctx->stream);
+    if (e == hipSuccess && total)
+        e = hipMemcpyAsync(ds->d_elems, elements, total * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && k) e = hipMemcpyAsync(ds->d_sizes, sizes, uint64_t(k) * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("dataset upload: ") + hipGetErrorString(e));
+    {
+        KernelScope ks(ctx, "offsets_scan");
+        hipLaunchKernelGGL(k_offsets_scan, dim3(1), dim3(1024), 0, ctx->stream, ds->d_sizes, k, ds->d_offsets);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("offsets scan: ") + hipGetErrorString(e));
+    // the largest element (sets are sorted: the max of the last elements)
+    for (uint32_t i = 0; i < k; ++i)
+        if (sizes[i]) ds->max_elem = std::max(ds->max_elem, elements[ds->offsets[i + 1] - 1]);
+    if (validate && total > 1) {
+        void* ws = nullptr;
+        int st = small_workspace(ctx, 256, &ws);
+        if (st != SI_OK) return cleanup(st, si_last_error());
+        uint32_t* bad = static_cast<uint32_t*>(ws) + 32;  // slot reserved for validation
+        uint32_t grid = uint32_t(std::min<uint64_t>((total + 255) / 256, uint64_t(ctx->num_cus) * 16));
+        hipLaunchKernelGGL(k_check_sorted, dim3(grid), dim3(256), 0, ctx->stream, ds->d_elems, ds->d_offsets, k, total,
+                           bad);
+        uint32_t nbad = 0;
+        e = hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(bad, 0, 4, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("validation: ") + hipGetErrorString(e));
+        if (nbad) return cleanup(SI_ERR_UNSORTED, std::to_string(nbad) + " elements are not strictly ascending within their set");
+    }
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("dataset upload: ") + hipGetErrorString(e));
+    *out = ds;
+    return SI_OK;
+}
+
+int si_dataset_free(si_dataset* ds) {
+    if (!ds) return SI_OK;
+    (void)hipStreamSynchronize(ds->ctx->stream);
+    if (ds->d_elems) (void)hipFree(ds->d_elems);
+    if (ds->d_sizes) (void)hipFree(ds->d_sizes);
+    if (ds->d_offsets) (void)hipFree(ds->d_offsets);
+    delete ds;
+    return SI_OK;
+}
+
+int si_dataset_info(si_dataset* ds, uint32_t* k, uint32_t* universe, uint64_t* total) {
+    if (!ds) return fail(SI_ERR_ARG, "null dataset");
+    if (k) *k = ds->k;
+    if (universe) *universe = ds->universe;
+    if (total) *total = ds->total;
+    return SI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""GPU parity of the merge-path (ip) kernels against the CPU oracle.
+
+Counts and materialised lists must be bit-exact.  Sizes are chosen to hit
+tile (4096 merged elements), wave (64) and segment boundaries, empty and
+singleton sets, values straddling 2^31 and the top of the uint32 range.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import set_intersection_amd as si
+from set_intersection_amd import datasets
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def check_pair(ctx, a, b, universe=None):
+    coll = si.Collection.from_sets([a, b], universe=universe)
+    ds = ctx.upload(coll)
+    exp = oracle.materialize(coll.set(0), coll.set(1))
+    assert ds.pair_count(0, 1) == exp.size
+    assert ds.pair_count(1, 0) == exp.size
+    got = ds.pair_materialize(0, 1)
+    assert np.array_equal(got, exp)
+    got = ds.pair_materialize(1, 0)
+    assert np.array_equal(got, exp)
+    ds.free()
+
+
+def test_closed_form_multiples(ctx):
+    for N, p, q in [(100_000, 2, 3), (1_000_003, 4, 6), (50_000, 7, 7), (300_000, 1, 5), (2_000_000, 97, 89)]:
+        a = np.arange(0, N, p, dtype=np.uint32)
+        b = np.arange(0, N, q, dtype=np.uint32)
+        check_pair(ctx, a, b)
+
+
+def test_edge_sets(ctx):
+    top = np.array([0, 1, 2**31 - 1, 2**31, 2**31 + 1, 2**32 - 2, 2**32 - 1], dtype=np.uint32)
+    empty = np.zeros(0, dtype=np.uint32)
+    check_pair(ctx, top, top)
+    check_pair(ctx, top, empty)
+    check_pair(ctx, empty, empty)
+    check_pair(ctx, top, top[[0, 3, 6]])
+    check_pair(ctx, np.array([5], np.uint32), np.array([5], np.uint32))
+    check_pair(ctx, np.array([5], np.uint32), np.array([6], np.uint32))
+    # disjoint interleaved and disjoint ranges
+    check_pair(ctx, np.arange(0, 20000, 2, dtype=np.uint32), np.arange(1, 20000, 2, dtype=np.uint32))
+    check_pair(ctx, np.arange(0, 9000, dtype=np.uint32), np.arange(9000, 30000, dtype=np.uint32))
+    # all of A before all of B except one shared element at the seam
+    check_pair(ctx, np.arange(0, 9001, dtype=np.uint32), np.arange(9000, 30000, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("na,nb", [(63, 64), (64, 65), (4095, 1), (4096, 4096), (4097, 4095), (8191, 8193),
+                                   (1, 100_000), (100_000, 3), (123_457, 98_765)])
+def test_boundary_sizes(ctx, na, nb):
+    rng = np.random.default_rng(na * 7 + nb)
+    U = max(na, nb) * 3
+    a = datasets.uniform_sorted(na, U, 11 + na)
+    b = datasets.uniform_sorted(nb, U, 13 + nb)
+    # plant matches so that even sparse draws intersect
+    if na and nb:
+        extra = rng.choice(a, min(na, nb, 50), replace=False)
+        b = np.union1d(b, extra).astype(np.uint32)
+    check_pair(ctx, a, b)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_ratios(ctx, seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(6):
+        na = int(rng.integers(0, 300_000))
+        nb = int(rng.integers(0, 300_000))
+        U = int(rng.choice([na + nb + 1, 4 * (na + nb) + 7, 2**32]))
+        a = datasets.uniform_sorted(min(na, U), U, int(rng.integers(1 << 30)))
+        b = datasets.uniform_sorted(min(nb, U), U, int(rng.integers(1 << 30)))
+        check_pair(ctx, a, b)
+
+
+def test_sisi_2p20_in_2p24(ctx):
+    """The CPU-runnable SISI workload: |A|=|B|=2^20 uniform from [0,2^24)."""
+    coll = datasets.make_workload("sisi_2^20_in_2^24")
+    ds = ctx.upload(coll)
+    exp = oracle.materialize(coll.set(0), coll.set(1))
+    assert 60_000 < exp.size < 71_000  # E = 65,536
+    assert ds.pair_count(0, 1) == exp.size
+    assert np.array_equal(ds.pair_materialize(0, 1), exp)
+
+
+def test_sisi_large_oracle(ctx):
+    """2^25 x 2^25 from [0,2^27): many segments per launch, oracle-checked."""
+    coll = datasets.sisi_pair(1 << 25, 1 << 25, 1 << 27, 99)
+    ds = ctx.upload(coll)
+    exp = oracle.materialize(coll.set(0), coll.set(1))
+    assert ds.pair_count(0, 1) == exp.size
+    assert np.array_equal(ds.pair_materialize(0, 1), exp)
+
+
+def test_async_buffers_and_repeat(ctx):
+    coll = datasets.sisi_pair(300_000, 200_000, 1 << 21, 5)
+    ds = ctx.upload(coll)
+    exp = oracle.materialize(coll.set(0), coll.set(1))
+    out = ctx.buffer(8 * 4)
+    for i in range(4):  # repeated launches reuse the self-clearing ticket counter
+        ds.pair_count_async(0, 1, out, offset=8 * i)
+    ctx.sync()
+    assert out.download(np.uint64).tolist() == [exp.size] * 4
+    lst = ctx.buffer(200_000 * 4)
+    cnt = ctx.buffer(8)
+    ds.pair_materialize_async(0, 1, lst, cnt)
+    n = int(cnt.download(np.uint64)[0])
+    assert n == exp.size
+    assert np.array_equal(lst.download(np.uint32, count=n), exp)
+
+
+def test_errors(ctx):
+    coll = si.Collection.from_sets([[1, 2, 3], [2, 3, 4]])
+    ds = ctx.upload(coll)
+    with pytest.raises(si.SIError):
+        ds.pair_count(0, 5)
+    with pytest.raises(si.SIError) as e:
+        ctx.upload(si.Collection.from_sets([[1, 3, 2]]))
+    assert e.value.status == -5  # SI_ERR_UNSORTED
+    with pytest.raises(si.SIError):
+        ctx.upload(si.Collection.from_sets([[1, 1, 2]]))
+
+
+# ------------------------------------------------------------------ MISI --
+def run_layout_counts(ds, coll, technique, partition, p=None):
+    """All pair counts via the reference's multi-run driver layout."""
+    k = coll.k
+    tiles = [(t, t * partition, min((t + 1) * partition, k)) for t in range((k + partition - 1) // partition)]
+    single = len(tiles) == 1
+    n = k if single else partition
+    res = {}
+    for x in range(len(tiles)):
+        for y in range(x, len(tiles)):
+            A, B = si.tile(*tiles[x]), si.tile(*tiles[y])
+            c = ds.run_count(technique, A, B, n, p)
+            self_join = A.id == B.id
+            for i in range(A.start, A.end):
+                for j in range(i + 1 if self_join else B.start, B.end):
+                    li, lj = i - A.id * n, j - B.id * n
+                    idx = oracle.triangular_index(n, li, lj) if self_join else li * n + lj
+                    res[(i, j)] = int(c[idx])
+    return res
+
+
+@pytest.mark.parametrize("name", ["closed_form", "random_k12", "random_k23"])
+@pytest.mark.parametrize("partition", [3, 5, 10000])
+def test_golden_runs(ctx, name, partition):
+    coll = datasets.read_bin(os.path.join(GOLDEN, name + ".bin"))
+    ds = ctx.upload(coll)
+    got = run_layout_counts(ds, coll, si.TECH_IP, partition)
+    counts, _ = oracle.all_pairs(coll)
+    k = coll.k
+    for (i, j), c in got.items():
+        assert c == counts[oracle.triangular_index(k, i, j)], (i, j)
+    assert len(got) == k * (k - 1) // 2
+
+
+def test_runs_random_collection(ctx):
+    coll = datasets.zipf_collection(300, 1.2, 4, 5000, 1 << 16, 3)
+    ds = ctx.upload(coll)
+    counts, _ = oracle.all_pairs(coll)
+    got = run_layout_counts(ds, coll, si.TECH_IP, 128)
+    k = coll.k
+    for (i, j), c in got.items():
+        assert c == counts[oracle.triangular_index(k, i, j)]
