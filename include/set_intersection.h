@@ -209,39 +209,60 @@ int si_mm_counts_async(si_dataset* ds, si_tile A, si_tile B, int32_t variant, ui
 /* -------------------------------------------------------- batched MISI -- */
 /* All k(k-1)/2 pair counts of the whole collection in triangular order
  * (counts[triangular_index(k, a, b)], the single-run layout of
- * include/io.hpp:91-102), by the size-binned persistent work-queue kernel that
- * picks merge path, binary search or bitmap probing per pair.  Has no single
- * reference counterpart: it replaces the per-run loops of ip/obs/hi/
- * bitmap_dynamic for k > partition.  `tuning` may be NULL (built-in
- * defaults) or a tuning record produced by si_batched_autotune. */
+ * include/io.hpp:91-102).  Has no single reference counterpart: it replaces
+ * the per-run loops of ip/obs/hi/bitmap_dynamic (ip.cu:141-177 and the same
+ * loop in obs.cu, hi.cu, bitmap_dynamic.cu) for a whole collection.
+ *
+ * Sets are ordered by size (descending) and every pair is charged to its
+ * larger set ("row").  Rows are binned by size and size ratio, and each bin
+ * is dispatched to one technique by persistent work-queue kernels:
+ *   BITMAP  rows with |S| >= bitmap_min whose universe fits an LDS bitmap;
+ *   HASH    rows whose LDS hash table fits (2|S| slots);
+ *   SEARCH  larger rows: partners with |row|/|partner| >= ratio_search
+ *           binary-search the row (LDS sample cache + global search);
+ *   MERGE   the remaining pairs of large rows (merge path).
+ * Each work item is (row, range of smaller partner sets); the partner
+ * elements are streamed once per row and probed, and per-pair counts are
+ * reduced with wave ballots.  `tuning` may be NULL (defaults) or a record from
+ * si_batched_autotune. */
 typedef struct si_tuning {
-    uint32_t version;           /* must be SI_TUNING_VERSION                    */
-    uint32_t small_max;         /* sets with |S| <= small_max are "small"       */
-    uint32_t ratio_search;      /* |L|/|S| >= ratio_search -> binary search     */
-    uint32_t bitmap_min;        /* rows with |S| >= bitmap_min use LDS bitmaps  */
-    uint32_t chunk_pairs;       /* pairs per work-queue item                    */
-    uint32_t wg_per_cu;         /* persistent workgroups per CU                 */
-    uint32_t use_graph;         /* capture the launches in a hipGraph           */
+    uint32_t version;            /* must be SI_TUNING_VERSION                     */
+    uint32_t bitmap_min;         /* rows with |S| >= bitmap_min use LDS bitmaps   */
+    uint32_t ratio_search;       /* SEARCH when |row| >= ratio_search * |partner| */
+    uint32_t chunk_elems;        /* partner elements per work item (hash rows)    */
+    uint32_t chunk_elems_bitmap; /* partner elements per work item (bitmap rows)  */
+    uint32_t probe_wg_per_cu;    /* persistent workgroups per CU (hash rows)      */
+    uint32_t use_graph;          /* replay the launches from a captured hipGraph  */
     uint32_t reserved[9];
 } si_tuning;
-#define SI_TUNING_VERSION 1u
+#define SI_TUNING_VERSION 2u
 int si_batched_default_tuning(si_dataset* ds, si_tuning* out);
 int si_batched_count(si_dataset* ds, const si_tuning* tuning, uint32_t* counts_host);
 int si_batched_count_async(si_dataset* ds, const si_tuning* tuning, si_buffer* counts);
-/* Times candidate settings on this dataset and returns the fastest record.
- * budget_ms bounds the total measuring time. */
+/* Times candidate settings on this dataset (each candidate run once, until
+ * budget_ms of measured time is spent) and returns the fastest record. */
 int si_batched_autotune(si_dataset* ds, double budget_ms, si_tuning* out);
 
 /* ------------------------------------------------------ edge-wise counts -- */
 /* For every edge e = (u[e], v[e]) of a graph whose adjacency lists are the
  * sets of `ds` (one set per vertex), counts[e] = |N(u) ∩ N(v)|, plus the
- * uint64 sum over all edges.  Uses the batched per-pair dispatcher. */
+ * uint64 sum over all edges (3x the triangle count when every undirected
+ * edge is listed once).  No reference counterpart: the reference intersects
+ * all pairs of a collection; an edge list is an explicit pair list handled by
+ * the same binned dispatcher (each edge is charged to its endpoint with the
+ * larger list, whose bitmap / hash table / sample cache is built once in LDS
+ * and probed by the smaller lists of all its edges).
+ * si_edge_plan groups the edges (host arrays, O(m)); the plan can be run any
+ * number of times. */
+typedef struct si_edge_plan si_edge_plan;
+int si_edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m,
+                        const si_tuning* tuning, si_edge_plan** out);
+int si_edge_plan_free(si_edge_plan* plan);
+/* counts: m uint32 (device); total: uint64 at byte offset 0 of total_out. */
+int si_edge_count_async(si_edge_plan* plan, si_buffer* counts, si_buffer* total_out);
+/* One-shot form: plan, run, download, free. */
 int si_edge_count(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m,
                   const si_tuning* tuning, uint32_t* counts_host, uint64_t* total);
-/* Device form: u, v, counts live in device buffers; total written as uint64 at
- * byte offset 0 of `total_out`. */
-int si_edge_count_async(si_dataset* ds, si_buffer* u, si_buffer* v, uint64_t m,
-                        const si_tuning* tuning, si_buffer* counts, si_buffer* total_out);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,33 @@ class Buffer:
             pass
 
 
+class EdgePlan:
+    """Edges grouped for the binned dispatcher (si_edge_plan_create)."""
+
+    def __init__(self, ds, u, v, tuning=None):
+        self.ds = ds
+        u = np.ascontiguousarray(u, dtype=np.uint32)
+        v = np.ascontiguousarray(v, dtype=np.uint32)
+        self.m = int(u.size)
+        self.h = c_void_p()
+        _check(_abi.lib().si_edge_plan_create(ds.h, _u32p(u), _u32p(v), u.size, byref(tuning) if tuning else None,
+                                              byref(self.h)))
+
+    def count_async(self, counts, total):
+        _check(_abi.lib().si_edge_count_async(self.h, counts.h, total.h))
+
+    def free(self):
+        if self.h:
+            _abi.lib().si_edge_plan_free(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU and one HIP stream (si_open)."""
 
@@ -230,9 +257,8 @@ class Dataset:
                                         _u32p(counts), byref(total)))
         return counts[: u.size], total.value
 
-    def edge_count_async(self, u_buf, v_buf, m, counts, total, tuning=None):
-        _check(_abi.lib().si_edge_count_async(self.h, u_buf.h, v_buf.h, m, byref(tuning) if tuning else None,
-                                              counts.h, total.h))
+    def edge_plan(self, u, v, tuning=None):
+        return EdgePlan(self, u, v, tuning)
 
     def free(self):
         if self.h:

@@ -21,7 +21,7 @@ TECH_IP, TECH_OBS, TECH_HI, TECH_BITMAP_NAIVE, TECH_BITMAP_DYNAMIC, TECH_MM = ra
 TECHNIQUES = {"ip": TECH_IP, "obs": TECH_OBS, "hi": TECH_HI, "bitmap_naive": TECH_BITMAP_NAIVE,
               "bitmap_dynamic": TECH_BITMAP_DYNAMIC, "mm": TECH_MM}
 MM_I8, MM_F32, MM_POPCOUNT = 0, 1, 2
-SI_TUNING_VERSION = 1
+SI_TUNING_VERSION = 2
 
 
 class si_params(Structure):
@@ -41,9 +41,12 @@ class si_device_info(Structure):
 
 
 class si_tuning(Structure):
-    _fields_ = [("version", c_uint32), ("small_max", c_uint32), ("ratio_search", c_uint32),
-                ("bitmap_min", c_uint32), ("chunk_pairs", c_uint32), ("wg_per_cu", c_uint32),
+    _fields_ = [("version", c_uint32), ("bitmap_min", c_uint32), ("ratio_search", c_uint32),
+                ("chunk_elems", c_uint32), ("chunk_elems_bitmap", c_uint32), ("probe_wg_per_cu", c_uint32),
                 ("use_graph", c_uint32), ("reserved", c_uint32 * 9)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
 P = c_void_p  # opaque handles
@@ -86,8 +89,10 @@ SIGNATURES = {
     "si_batched_count": (c_int, [P, POINTER(si_tuning), U32P]),
     "si_batched_count_async": (c_int, [P, POINTER(si_tuning), P]),
     "si_batched_autotune": (c_int, [P, c_double, POINTER(si_tuning)]),
+    "si_edge_plan_create": (c_int, [P, U32P, U32P, c_uint64, POINTER(si_tuning), POINTER(P)]),
+    "si_edge_plan_free": (c_int, [P]),
+    "si_edge_count_async": (c_int, [P, P, P]),
     "si_edge_count": (c_int, [P, U32P, U32P, c_uint64, POINTER(si_tuning), U32P, U64P]),
-    "si_edge_count_async": (c_int, [P, P, P, c_uint64, POINTER(si_tuning), P, P]),
 }
 
 GEN_SIGNATURES = {

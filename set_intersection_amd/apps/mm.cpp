@@ -1,0 +1,137 @@
+// mm -- bitmap-GEMM set intersection; drop-in for the reference's
+// src/matrix_multiplication.cu executable (target name `mm`, CMakeLists.txt:152).
+// Same flags (--input, --output, --partition), tables, tile loop (:139-184) and
+// result file (writeResult<T, true>, io.hpp:104-136).  The incidence matrices
+// are built on the device and multiplied with int8 MFMA (int32 accumulation);
+// --variant f32 selects the FP32-accumulate comparison path (exact below 2^24),
+// --variant popcount the bitmap AND + popcount path.
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "cli.hpp"
+#include "format.hpp"
+#include "set_intersection.h"
+#include "tiles.hpp"
+#include "timers.hpp"
+
+static int die(si_context* ctx) {
+    std::fprintf(stderr, "GPUassert: %s\n", si_last_error());
+    if (ctx) si_close(ctx);
+    return 1;
+}
+
+int main(int argc, char** argv) {
+    si::printTitle("GPU Matrix multiplication set intersection");
+    si_context* ctx = nullptr;
+    if (si_open(0, &ctx) != SI_OK) return die(nullptr);
+    si_device_info info;
+    si_device_info_get(ctx, &info);
+    std::string input, output, variant = "i8";
+    uint32_t partition = 10000;
+    si::Options options(argv[0], "Help");
+    options.add("input", "Input dataset path", &input)
+        .add("output", "Output result path", &output)
+        .add("partition", "Number of sets to be processed per GPU invocation", &partition)
+        .add("variant", "i8 (int8 MFMA, default) | f32 (FP32 accumulate) | popcount", &variant)
+        .addFlag("help", "Print help");
+    try {
+        options.parse(argc, argv);
+    } catch (const si::CliError& e) {
+        std::printf("%s\n", e.what());
+        si_close(ctx);
+        return 1;
+    }
+    if (options.count("help")) {
+        std::printf("%s\n", options.help().c_str());
+        si_close(ctx);
+        return 0;
+    }
+    if (!options.count("input")) {
+        std::printf("%s\n", "No input dataset given! Exiting...");
+        si_close(ctx);
+        return 1;
+    }
+    int32_t v = variant == "f32" ? SI_MM_F32 : variant == "popcount" ? SI_MM_POPCOUNT : variant == "i8" ? SI_MM_I8 : -1;
+    if (v < 0 || partition == 0) {
+        std::printf("bad --variant or --partition\n");
+        si_close(ctx);
+        return 1;
+    }
+    si::Dataset d;
+    try {
+        d = si::readDataset(input);
+    } catch (const si::FormatError& e) {
+        std::printf("%s\n", e.what());
+        si_close(ctx);
+        return 1;
+    }
+    si::printTable("Dataset characteristics", {{"Cardinality", si::toText(d.cardinality)},
+                                               {"Universe", si::toText(d.universe)},
+                                               {"Total elements", si::toText(d.totalElements)}});
+    partition = std::min(d.cardinality, partition);
+    if (partition == 0) partition = 1;
+    std::vector<si::tile> tiles = si::splitToTiles(d.cardinality, partition);
+    std::vector<si::tile_pair> runs = si::findTilePairs(d.cardinality, partition);
+    const uint64_t combinations = uint64_t(partition) * partition;
+    const uint64_t K = uint64_t(d.universe) + 1;
+    const uint64_t deviceMemory = (K * partition * 2) + combinations * 4;
+    si::printTable("Launch info", {{"Partition", si::toText(partition)},
+                                   {"GPU invocations", si::toText(runs.size())},
+                                   {"Required memory (Output)", si::formatBytes(runs.size() * combinations * 4)},
+                                   {"Required memory (GPU)", si::formatBytes(deviceMemory)}});
+    if (deviceMemory > info.free_global_mem) {
+        std::printf("Error not enough GPU memory (%s)!\nExiting...", si::formatBytes(info.free_global_mem).c_str());
+        si_close(ctx);
+        return 1;
+    }
+    si::HostTimer hostTimer;
+    si::DeviceTimer deviceTimer(ctx);
+    si::Interval* hostAlloc = hostTimer.add("Allocate memory");
+    std::vector<uint32_t> counts(runs.size() * combinations);
+    si::HostTimer::finish(hostAlloc);
+
+    si::EventPair* devAlloc = deviceTimer.add("Device memory allocation");
+    si_buffer* dOut = nullptr;
+    if (si_buffer_alloc(ctx, combinations * 4, &dOut) != SI_OK) return die(ctx);
+    si::DeviceTimer::finish(devAlloc);
+    si::EventPair* xfer = deviceTimer.add("Transfer to device");
+    si_dataset* ds = nullptr;
+    if (si_dataset_upload(ctx, d.cardinality, d.universe, d.totalElements, d.sizes.data(), d.elements.data(), 1, &ds) !=
+        SI_OK)
+        return die(ctx);
+    si::DeviceTimer::finish(xfer);
+
+    uint64_t iter = 0;
+    for (size_t i = 0; i < tiles.size(); ++i) {
+        for (size_t j = i; j < tiles.size(); ++j) {
+            const si::tile& A = tiles[i];
+            const si::tile& B = tiles[j];
+            si::EventPair* mmEv = deviceTimer.add("Matrix multiplication");
+            if (si_mm_counts_async(ds, si_tile{A.id, A.start, A.end}, si_tile{B.id, B.start, B.end}, v, partition,
+                                   dOut) != SI_OK)
+                return die(ctx);
+            si::DeviceTimer::finish(mmEv);
+            si::EventPair* tr = deviceTimer.add("Transfer result");
+            const uint64_t used = uint64_t(A.length - 1) * partition + B.length;
+            if (si_buffer_download(dOut, 0, counts.data() + iter * combinations, used * 4) != SI_OK) return die(ctx);
+            si::DeviceTimer::finish(tr);
+            ++iter;
+        }
+    }
+    si::EventPair* fr = deviceTimer.add("Free device memory");
+    si_dataset_free(ds);
+    si_buffer_free(dOut);
+    si::DeviceTimer::finish(fr);
+    si_sync(ctx);
+    hostTimer.print();
+    deviceTimer.print();
+    if (!output.empty()) {
+        std::printf("Writing result to file %s\n", output.c_str());
+        si::writeResult<uint32_t, true>(runs, partition, counts, output);
+        std::printf("Finished\n");
+    }
+    si_close(ctx);
+    return 0;
+}

@@ -1,0 +1,886 @@
+// Batched all-pairs (MISI) dispatcher for gfx950.
+//
+// Replaces the reference's per-run loops (ip.cu:141-177; the same loop in
+// obs.cu:173-243, hi.cu:198-261, bitmap_dynamic.cu:363-389), which give every
+// pair of a run the same kernel and re-read both sets for every pair.
+//
+// Design:
+//  * Plan (once per dataset): sets are ordered by size, descending, and the
+//    elements are gathered into that order on the device.  Every pair (i, j),
+//    i < j in that order, is charged to its larger set i (the "row"); the
+//    partners of row i are the sets i+1 .. k-1, whose elements are ONE
+//    contiguous range of the reordered element array.
+//  * Bins: a row is BITMAP (|S_i| >= bitmap_min and the universe fits a
+//    2^20-bit LDS bitmap), HASH (an LDS open-addressing table of 2|S_i|
+//    slots fits) or BIG.  For a BIG row the partners with
+//    |S_i| >= ratio_search * |S_j| are SEARCH partners (binary search in S_i,
+//    first level from an LDS sample cache) and the rest are MERGE pairs (merge
+//    path).  So each pair costs about min(|S_i|, |S_j|) LDS probes instead of
+//    |S_i| + |S_j| merge steps.
+//  * Work items are (row, partner-set range) cut at set boundaries into
+//    ~chunk_elems partner elements.  Persistent workgroups pop items from an
+//    atomic queue; a workgroup rebuilds its row structure only when the row
+//    changes.  Inside an item the waves split the partner elements evenly,
+//    stream them with coalesced loads, probe, and turn the hit bits into
+//    per-pair counts with wave ballots + popcount over each set's bit range;
+//    a set cut by a wave boundary is completed through LDS.  Every count is
+//    written exactly once with a plain store, so the output needs no clearing
+//    and the launch sequence (queue reset + kernels) can be captured in a
+//    hipGraph and replayed.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "si_device.h"
+#include "si_internal.h"
+#include "si_merge.h"
+
+namespace {
+
+using namespace si_dev;
+
+enum : uint32_t { MODE_BITMAP = 0, MODE_HASH = 1, MODE_SEARCH = 2 };
+
+struct Item {
+    uint32_t row, j0, j1, mode;
+};
+
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+constexpr int BM_NT = 1024, BM_WORDS = 32768;  // 128 KiB LDS bitmap: universes up to 2^20
+constexpr int PR_NT = 256, PR_WORDS = 8192;    // 32 KiB LDS: hash tables of <= 8192 slots
+constexpr uint32_t HASH_MAX = PR_WORDS / 2;    // largest row that fits a hash table
+constexpr int SEARCH_NS = 4096;                // samples cached for SEARCH rows
+constexpr int VT = 8;                          // elements per lane per chunk
+
+__device__ __forceinline__ uint32_t hslot(uint32_t x, uint32_t logTS) { return (x * 0x9E3779B1u) >> (32 - logTS); }
+
+__device__ __forceinline__ uint64_t pair_slot(uint32_t k, uint32_t pa, uint32_t pb) {
+    const uint32_t lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
+    return tri_index(k, lo, hi);
+}
+
+// bits [lo, hi) of a 64-bit word (0 <= lo <= hi <= 64)
+__device__ __forceinline__ uint64_t bit_range(int lo, int hi) {
+    const uint64_t h = hi >= 64 ? ~0ull : ((1ull << hi) - 1);
+    const uint64_t l = lo >= 64 ? ~0ull : ((1ull << lo) - 1);
+    return h & ~l;
+}
+
+template <int LDSW>
+struct RowSmem {
+    uint32_t tab[LDSW];
+    uint32_t item, row, mode, hasMax, logTS;
+    uint32_t pset[2 * 32], pcnt[2 * 32];
+};
+
+template <int NT, int LDSW>
+__device__ void build_row(RowSmem<LDSW>& sm, const uint32_t* __restrict__ S, uint64_t n, uint32_t mode,
+                          uint32_t bmWords) {
+    if (mode == MODE_BITMAP) {
+        for (uint32_t i = threadIdx.x; i < bmWords; i += NT) sm.tab[i] = 0;
+        __syncthreads();
+        for (uint64_t i = threadIdx.x; i < n; i += NT) {
+            const uint32_t x = S[i];
+            atomicOr(&sm.tab[x >> 5], 1u << (x & 31));
+        }
+    } else if (mode == MODE_HASH) {
+        uint32_t logTS = 6;
+        while ((1ull << logTS) < 2 * n) ++logTS;
+        const uint32_t TS = 1u << logTS;
+        for (uint32_t i = threadIdx.x; i < TS; i += NT) sm.tab[i] = EMPTY;
+        if (threadIdx.x == 0) {
+            sm.hasMax = 0;
+            sm.logTS = logTS;
+        }
+        __syncthreads();
+        for (uint64_t i = threadIdx.x; i < n; i += NT) {
+            const uint32_t x = S[i];
+            if (x == EMPTY) {
+                sm.hasMax = 1;
+                continue;
+            }
+            uint32_t s = hslot(x, logTS);
+            while (true) {
+                const uint32_t prev = atomicCAS(&sm.tab[s], EMPTY, x);
+                if (prev == EMPTY || prev == x) break;
+                s = (s + 1) & (TS - 1);
+            }
+        }
+    } else {  // SEARCH: sample cache
+        for (uint32_t t = threadIdx.x; t < SEARCH_NS; t += NT) sm.tab[t] = S[(uint64_t(t) * n) / SEARCH_NS];
+    }
+    __syncthreads();
+}
+
+template <int LDSW>
+__device__ __forceinline__ uint32_t probe(const RowSmem<LDSW>& sm, uint32_t mode, uint32_t x,
+                                          const uint32_t* __restrict__ S, uint64_t nS, uint32_t logTS, uint32_t hasMax) {
+    if (mode == MODE_BITMAP) return (sm.tab[x >> 5] >> (x & 31)) & 1u;
+    if (mode == MODE_HASH) {
+        if (x == EMPTY) return hasMax;
+        const uint32_t mask = (1u << logTS) - 1;
+        uint32_t s = hslot(x, logTS);
+        while (true) {
+            const uint32_t k = sm.tab[s];
+            if (k == x) return 1;
+            if (k == EMPTY) return 0;
+            s = (s + 1) & mask;
+        }
+    }
+    // SEARCH
+    uint32_t lo = 0, hi = SEARCH_NS;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sm.tab[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    if (lo == 0) return 0;
+    if (sm.tab[lo - 1] == x) return 1;
+    uint64_t l = (uint64_t(lo - 1) * nS) / SEARCH_NS + 1;
+    uint64_t h = (lo == SEARCH_NS) ? nS : (uint64_t(lo) * nS) / SEARCH_NS;
+    while (l < h) {
+        const uint64_t mid = (l + h) >> 1;
+        const uint32_t y = S[mid];
+        if (y == x) return 1;
+        if (y < x) l = mid + 1; else h = mid;
+    }
+    return 0;
+}
+
+// Persistent row kernel: pops items from queue[0], builds the row's LDS
+// structure, streams the item's partner elements and stores one count per pair.
+template <int NT, int LDSW>
+__global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uint32_t nitems, uint32_t* __restrict__ queue,
+                                             const uint32_t* __restrict__ selem, const uint64_t* __restrict__ soff,
+                                             const uint32_t* __restrict__ perm, uint32_t k, uint32_t bmWords,
+                                             uint32_t* __restrict__ out) {
+    constexpr int NW = NT / WAVE;
+    static_assert(NW <= 32, "partial slots");
+    __shared__ RowSmem<LDSW> sm;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+    if (tid == 0) {
+        sm.row = EMPTY;
+        sm.mode = EMPTY;
+    }
+    __syncthreads();
+    while (true) {
+        if (tid == 0) sm.item = atomicAdd(queue, 1u);
+        if (tid < 2 * NW) sm.pset[tid] = EMPTY;
+        __syncthreads();
+        const uint32_t it = sm.item;
+        if (it >= nitems) break;
+        const Item I = items[it];
+        if (I.row != sm.row || I.mode != sm.mode) {
+            __syncthreads();  // everyone has read sm.row / sm.mode
+            build_row<NT, LDSW>(sm, selem + soff[I.row], soff[I.row + 1] - soff[I.row], I.mode, bmWords);
+            if (tid == 0) {
+                sm.row = I.row;
+                sm.mode = I.mode;
+            }
+        }
+        const uint32_t mode = I.mode, logTS = sm.logTS, hasMax = sm.hasMax;
+        const uint64_t rowOff = soff[I.row], rowN = soff[I.row + 1] - rowOff;
+        const uint32_t* S = selem + rowOff;
+        const uint32_t prow = perm[I.row];
+
+        // this wave's share of the item's partner elements
+        const uint64_t e0 = soff[I.j0], e1 = soff[I.j1], len = e1 - e0;
+        const uint64_t ws = e0 + (len * w) / NW, we = e0 + (len * (w + 1)) / NW;
+        if (ws < we) {
+            // first set: largest j in [j0, j1) with soff[j] <= ws
+            uint32_t lo = I.j0, hi = I.j1;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (soff[mid] <= ws) lo = mid; else hi = mid;
+            }
+            uint32_t jc = lo;
+            uint32_t carry = 0;
+            for (uint64_t c0 = ws; c0 < we; c0 += WAVE * VT) {
+                const uint64_t c1 = (c0 + WAVE * VT < we) ? c0 + WAVE * VT : we;
+                uint64_t M[VT];
+#pragma unroll
+                for (int v = 0; v < VT; ++v) {
+                    const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
+                    uint32_t hit = 0;
+                    if (p < c1) hit = probe<LDSW>(sm, mode, selem[p], S, rowN, logTS, hasMax);
+                    M[v] = __ballot(hit);
+                }
+                // sets overlapping [c0, c1): jc, jc+1, ...; lane t takes jc + g*64 + t
+                uint32_t nextJc = jc, nextCarry = 0;
+                for (uint32_t g = 0;; g += WAVE) {
+                    const uint32_t j = jc + g + lane;
+                    const bool valid = j < I.j1 && soff[j] < c1;
+                    const uint64_t valid_mask = __ballot(valid);
+                    if (!valid_mask) break;
+                    uint32_t cnt = 0;
+                    uint64_t sEnd = 0;
+                    bool endsHere = false;
+                    if (valid) {
+                        const uint64_t sBeg = soff[j];
+                        sEnd = soff[j + 1];
+                        const int64_t a = int64_t((sBeg > c0 ? sBeg : c0) - c0);
+                        const int64_t b = int64_t((sEnd < c1 ? sEnd : c1) - c0);
+#pragma unroll
+                        for (int v = 0; v < VT; ++v) {
+                            int lo2 = int(a - 64 * v), hi2 = int(b - 64 * v);
+                            lo2 = lo2 < 0 ? 0 : (lo2 > 64 ? 64 : lo2);
+                            hi2 = hi2 < 0 ? 0 : (hi2 > 64 ? 64 : hi2);
+                            if (hi2 > lo2) cnt += __popcll(M[v] & bit_range(lo2, hi2));
+                        }
+                        if (g == 0 && lane == 0) cnt += carry;
+                        endsHere = sEnd <= c1;
+                        if (endsHere) {
+                            if (sEnd <= we && sBeg >= ws) {
+                                out[pair_slot(k, prow, perm[j])] = cnt;
+                            } else if (sEnd <= we) {
+                                sm.pset[2 * w] = j;  // started before this wave
+                                sm.pcnt[2 * w] = cnt;
+                            }
+                        } else if (c1 == we) {
+                            sm.pset[2 * w + 1] = j;  // continues past this wave
+                            sm.pcnt[2 * w + 1] = cnt;
+                        }
+                    }
+                    // the last valid lane's set decides where the next chunk starts
+                    const int last = 63 - __builtin_clzll(valid_mask);
+                    const uint32_t lastJ = __shfl(j, last, WAVE);
+                    const uint32_t lastEnds = __shfl(uint32_t(endsHere), last, WAVE);
+                    const uint32_t lastCnt = __shfl(cnt, last, WAVE);
+                    if (lastEnds) {
+                        nextJc = lastJ + 1;
+                        nextCarry = 0;
+                    } else {
+                        nextJc = lastJ;
+                        nextCarry = lastCnt;
+                    }
+                    if (last < 63) break;
+                }
+                jc = nextJc;
+                carry = nextCarry;
+            }
+        }
+        __syncthreads();
+        // complete the sets cut by wave boundaries
+        if (tid == 0) {
+            for (int s = 0; s < 2 * NW; ++s) {
+                const uint32_t j = sm.pset[s];
+                if (j == EMPTY) continue;
+                uint32_t tot = 0;
+                for (int t = s; t < 2 * NW; ++t)
+                    if (sm.pset[t] == j) {
+                        tot += sm.pcnt[t];
+                        if (t != s) sm.pset[t] = EMPTY;
+                    }
+                out[pair_slot(k, prow, perm[j])] = tot;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// MERGE pairs by the streaming merge path, one workgroup per pair:
+// pairs[p] = (set a, set b, output slot low, output slot high).
+template <int NT, int MVT>
+__global__ __launch_bounds__(NT) void k_merge_pairs(const uint4* __restrict__ pairs, uint32_t npairs,
+                                                    const uint32_t* __restrict__ elems,
+                                                    const uint64_t* __restrict__ offsets, uint32_t* __restrict__ out) {
+    __shared__ MergeSmem<NT, MVT> sm;
+    for (uint32_t p = blockIdx.x; p < npairs; p += gridDim.x) {
+        const uint4 pr = pairs[p];
+        const uint64_t oA = offsets[pr.x], nA = offsets[pr.x + 1] - oA, oB = offsets[pr.y], nB = offsets[pr.y + 1] - oB;
+        uint64_t c = 0;
+        if (nA && nB) c = merge_segment<NT, MVT, false>(elems, oA, nA, oB, nB, 0, 0, nA, nB, sm, nullptr);
+        c = block_sum_u64<NT>(c, sm.red);
+        if (threadIdx.x == 0) out[(uint64_t(pr.w) << 32) | pr.z] = uint32_t(c);
+        __syncthreads();
+    }
+}
+
+// Zero counts for every pair whose smaller set (in sorted order) is empty.
+__global__ void k_zero_pairs(const uint32_t* __restrict__ perm, uint32_t k, uint32_t kne, uint32_t* __restrict__ out) {
+    for (uint32_t i = blockIdx.x; i < k; i += gridDim.x) {
+        const uint32_t j0 = (i + 1 > kne) ? i + 1 : kne;
+        const uint32_t pi = perm[i];
+        for (uint32_t j = j0 + threadIdx.x; j < k; j += blockDim.x) out[pair_slot(k, pi, perm[j])] = 0;
+    }
+}
+
+__global__ void k_gather(const uint32_t* __restrict__ elems, const uint64_t* __restrict__ offsets,
+                         const uint32_t* __restrict__ perm, const uint64_t* __restrict__ soff, uint32_t k,
+                         uint32_t* __restrict__ selem) {
+    for (uint32_t s = blockIdx.x; s < k; s += gridDim.x) {
+        const uint64_t src = offsets[perm[s]], dst = soff[s], n = soff[s + 1] - dst;
+        for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) selem[dst + i] = elems[src + i];
+    }
+}
+
+
+// Edge lists: items (row u, partner range [p0, p1) of plist); every partner
+// is one (set id, output index) pair.  Waves split the item's partners and
+// walk them in batches of 64: the batch's partner sets are concatenated
+// virtually (lane t owns [excl_t, incl_t)), each element position finds its
+// partner by a shuffle binary search, and each lane sums the hit bits of its
+// own range.  Each partner is handled by exactly one wave.
+template <int NT, int LDSW>
+__global__ __launch_bounds__(NT) void k_rows_list(const Item* __restrict__ items, uint32_t nitems,
+                                                  uint32_t* __restrict__ queue, const uint32_t* __restrict__ elems,
+                                                  const uint64_t* __restrict__ offsets, const uint2* __restrict__ plist,
+                                                  uint32_t bmWords, uint32_t* __restrict__ out) {
+    constexpr int NW = NT / WAVE;
+    __shared__ RowSmem<LDSW> sm;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+    if (tid == 0) {
+        sm.row = EMPTY;
+        sm.mode = EMPTY;
+    }
+    __syncthreads();
+    while (true) {
+        if (tid == 0) sm.item = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t it = sm.item;
+        if (it >= nitems) break;
+        const Item I = items[it];
+        const uint64_t rowOff = offsets[I.row], rowN = offsets[I.row + 1] - rowOff;
+        if (I.row != sm.row || I.mode != sm.mode) {
+            __syncthreads();
+            build_row<NT, LDSW>(sm, elems + rowOff, rowN, I.mode, bmWords);
+            if (tid == 0) {
+                sm.row = I.row;
+                sm.mode = I.mode;
+            }
+        }
+        const uint32_t mode = I.mode, logTS = sm.logTS, hasMax = sm.hasMax;
+        const uint32_t* S = elems + rowOff;
+        const uint32_t np = I.j1 - I.j0;
+        const uint32_t pa = I.j0 + uint32_t((uint64_t(np) * w) / NW), pb = I.j0 + uint32_t((uint64_t(np) * (w + 1)) / NW);
+        for (uint32_t q0 = pa; q0 < pb; q0 += WAVE) {
+            const uint32_t q = q0 + lane;
+            const bool valid = q < pb;
+            uint2 pe = make_uint2(0, 0);
+            uint64_t off = 0, sz = 0;
+            if (valid) {
+                pe = plist[q];
+                off = offsets[pe.x];
+                sz = offsets[pe.x + 1] - off;
+            }
+            uint64_t incl = sz;
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const uint64_t t = __shfl_up(incl, d, WAVE);
+                if (lane >= uint32_t(d)) incl += t;
+            }
+            const uint64_t excl = incl - sz;
+            const uint64_t tot = __shfl(incl, WAVE - 1, WAVE);
+            uint32_t cnt = 0;
+            for (uint64_t c0 = 0; c0 < tot; c0 += WAVE * VT) {
+                uint64_t M[VT];
+#pragma unroll
+                for (int v = 0; v < VT; ++v) {
+                    const uint64_t pos = c0 + uint64_t(v) * WAVE + lane;
+                    uint32_t hit = 0;
+                    // largest r with excl_r <= pos (all lanes take part in the shuffles)
+                    int lo = 0, hi = WAVE - 1;
+#pragma unroll
+                    for (int step = 0; step < 6; ++step) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        const uint64_t em = __shfl(excl, mid, WAVE);
+                        if (em <= pos) lo = mid; else hi = mid - 1;
+                    }
+                    const uint64_t er = __shfl(excl, lo, WAVE);
+                    const uint64_t orr = __shfl(off, lo, WAVE);
+                    if (pos < tot) hit = probe<LDSW>(sm, mode, elems[orr + (pos - er)], S, rowN, logTS, hasMax);
+                    M[v] = __ballot(hit);
+                }
+                if (valid && sz) {
+                    const int64_t a = int64_t(excl) - int64_t(c0), b = int64_t(incl) - int64_t(c0);
+#pragma unroll
+                    for (int v = 0; v < VT; ++v) {
+                        int64_t l2 = a - 64 * v, h2 = b - 64 * v;
+                        l2 = l2 < 0 ? 0 : (l2 > 64 ? 64 : l2);
+                        h2 = h2 < 0 ? 0 : (h2 > 64 ? 64 : h2);
+                        if (h2 > l2) cnt += __popcll(M[v] & bit_range(int(l2), int(h2)));
+                    }
+                }
+            }
+            if (valid) out[pe.y] = cnt;
+        }
+        __syncthreads();
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sum_counts(const uint32_t* __restrict__ c, uint64_t m,
+                                                   uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
+                                                   uint64_t* __restrict__ out) {
+    __shared__ uint64_t red[NT / WAVE];
+    uint64_t acc = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * NT + threadIdx.x; i < m; i += uint64_t(gridDim.x) * NT) acc += c[i];
+    grid_fold_u64<NT>(acc, partials, done, out, nullptr, red);
+}
+
+__global__ void k_reset_queues(uint32_t* q) {
+    if (threadIdx.x < 4) q[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host --
+namespace si_impl {
+
+struct BatchPlan {
+    uint32_t k = 0, kne = 0;
+    std::vector<uint32_t> perm, ssize;
+    std::vector<uint64_t> soff;
+    uint32_t* d_perm = nullptr;
+    uint64_t* d_soff = nullptr;
+    uint32_t* d_selem = nullptr;
+    uint32_t* d_queue = nullptr;
+    // items for the current tuning
+    si_tuning tuned{};
+    bool have_items = false;
+    Item* d_bm_items = nullptr;
+    Item* d_pr_items = nullptr;
+    uint4* d_merge = nullptr;
+    uint32_t n_bm = 0, n_pr = 0, n_merge = 0;
+    uint64_t items_cap_bm = 0, items_cap_pr = 0, merge_cap = 0;
+    // captured launch sequence
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    void* g_out = nullptr;
+    si_tuning g_tuning{};
+    ~BatchPlan() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : {(void*)d_perm, (void*)d_soff, (void*)d_selem, (void*)d_queue, (void*)d_bm_items,
+                        (void*)d_pr_items, (void*)d_merge})
+            if (p) (void)hipFree(p);
+    }
+};
+
+void free_batch_plan(BatchPlan* p) { delete p; }
+
+static bool same_tuning(const si_tuning& a, const si_tuning& b) {
+    return a.bitmap_min == b.bitmap_min && a.ratio_search == b.ratio_search && a.chunk_elems == b.chunk_elems &&
+           a.chunk_elems_bitmap == b.chunk_elems_bitmap && a.probe_wg_per_cu == b.probe_wg_per_cu &&
+           a.use_graph == b.use_graph;
+}
+
+static uint32_t bitmap_words(si_dataset* ds) { return uint32_t((uint64_t(ds->max_elem) >> 5) + 1); }
+
+int batched_default_tuning(si_dataset*, si_tuning* t) {
+    std::memset(t, 0, sizeof(*t));
+    t->version = SI_TUNING_VERSION;
+    t->bitmap_min = 2048;
+    t->ratio_search = 32;
+    t->chunk_elems = 65536;
+    t->chunk_elems_bitmap = 1u << 20;
+    t->probe_wg_per_cu = 4;
+    t->use_graph = 1;
+    return SI_OK;
+}
+
+static int check_tuning(const si_tuning& t) {
+    if (t.version != SI_TUNING_VERSION) return fail(SI_ERR_ARG, "tuning record has the wrong version");
+    if (t.chunk_elems == 0 || t.chunk_elems_bitmap == 0 || t.probe_wg_per_cu == 0 || t.ratio_search == 0)
+        return fail(SI_ERR_ARG, "tuning record has a zero field");
+    return SI_OK;
+}
+
+static int build_plan(si_dataset* ds) {
+    if (ds->plan) return SI_OK;
+    si_context* ctx = ds->ctx;
+    auto plan = std::make_unique<BatchPlan>();
+    const uint32_t k = ds->k;
+    plan->k = k;
+    plan->perm.resize(k);
+    std::iota(plan->perm.begin(), plan->perm.end(), 0u);
+    std::stable_sort(plan->perm.begin(), plan->perm.end(),
+                     [&](uint32_t a, uint32_t b) { return ds->sizes[a] > ds->sizes[b]; });
+    plan->ssize.resize(k);
+    plan->soff.assign(size_t(k) + 1, 0);
+    for (uint32_t i = 0; i < k; ++i) {
+        plan->ssize[i] = ds->sizes[plan->perm[i]];
+        plan->soff[i + 1] = plan->soff[i] + plan->ssize[i];
+    }
+    plan->kne = 0;
+    while (plan->kne < k && plan->ssize[plan->kne] > 0) ++plan->kne;
+    SI_HIP(hipMalloc(&plan->d_perm, std::max<uint64_t>(k, 1) * 4));
+    SI_HIP(hipMalloc(&plan->d_soff, (uint64_t(k) + 1) * 8));
+    SI_HIP(hipMalloc(&plan->d_selem, (ds->total + 64) * 4));
+    SI_HIP(hipMalloc(&plan->d_queue, 256));
+    SI_HIP(hipMemcpyAsync(plan->d_perm, plan->perm.data(), uint64_t(k) * 4, hipMemcpyHostToDevice, ctx->stream));
+    SI_HIP(hipMemcpyAsync(plan->d_soff, plan->soff.data(), (uint64_t(k) + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+    SI_HIP(hipMemsetAsync(plan->d_selem + ds->total, 0, 64 * 4, ctx->stream));
+    SI_HIP(hipMemsetAsync(plan->d_queue, 0, 256, ctx->stream));
+    if (k) {
+        KernelScope ks(ctx, "batched_gather");
+        hipLaunchKernelGGL(k_gather, dim3(std::min<uint32_t>(k, 65535)), dim3(256), 0, ctx->stream, ds->d_elems,
+                           ds->d_offsets, plan->d_perm, plan->d_soff, k, plan->d_selem);
+        SI_HIP(hipGetLastError());
+    }
+    SI_HIP(hipStreamSynchronize(ctx->stream));
+    ds->plan = plan.release();
+    return SI_OK;
+}
+
+// Builds the work items for `t` (host), uploads them.
+static int build_items(si_dataset* ds, const si_tuning& t) {
+    BatchPlan* P = ds->plan;
+    if (P->have_items && same_tuning(P->tuned, t)) return SI_OK;
+    si_context* ctx = ds->ctx;
+    const uint32_t kne = P->kne;
+    const bool bmFits = bitmap_words(ds) <= uint32_t(BM_WORDS);
+    std::vector<Item> bm, pr;
+    std::vector<uint4> merge;
+    auto emit = [&](std::vector<Item>& v, uint32_t row, uint32_t j0, uint32_t j1, uint32_t mode, uint64_t chunk) {
+        // split [j0, j1) at set boundaries into ~chunk partner elements
+        uint32_t j = j0;
+        while (j < j1) {
+            const uint64_t target = P->soff[j] + chunk;
+            uint32_t e = uint32_t(std::upper_bound(P->soff.begin() + j + 1, P->soff.begin() + j1 + 1, target) -
+                                  P->soff.begin()) - 1;
+            if (e <= j) e = j + 1;  // one set larger than the chunk
+            v.push_back(Item{row, j, e, mode});
+            j = e;
+        }
+    };
+    for (uint32_t i = 0; i + 1 < kne; ++i) {
+        const uint32_t si = P->ssize[i];
+        if (bmFits && (si >= t.bitmap_min || si > HASH_MAX)) {
+            emit(bm, i, i + 1, kne, MODE_BITMAP, t.chunk_elems_bitmap);
+        } else if (si <= HASH_MAX) {
+            emit(pr, i, i + 1, kne, MODE_HASH, t.chunk_elems);
+        } else {
+            // BIG row: partners with si >= ratio * |S_j| are searched, larger ones merged
+            uint32_t j = i + 1;
+            while (j < kne && uint64_t(P->ssize[j]) * t.ratio_search > si) {
+                const uint32_t pa = std::min(P->perm[i], P->perm[j]), pb = std::max(P->perm[i], P->perm[j]);
+                const uint64_t slot = (uint64_t(P->k) * (P->k - 1) / 2) - uint64_t(P->k - pa) * (P->k - pa - 1) / 2 + pb - pa - 1;
+                merge.push_back(make_uint4(P->perm[i], P->perm[j], uint32_t(slot), uint32_t(slot >> 32)));
+                ++j;
+            }
+            if (j < kne) emit(pr, i, j, kne, MODE_SEARCH, t.chunk_elems);
+        }
+    }
+    auto upload = [&](auto& vec, auto*& dptr, uint64_t& cap) -> int {
+        using T = typename std::decay<decltype(vec)>::type::value_type;
+        if (vec.size() > cap || !dptr) {
+            if (dptr) SI_HIP(hipFree(dptr));
+            dptr = nullptr;
+            cap = std::max<uint64_t>(vec.size(), 1);
+            SI_HIP(hipMalloc(reinterpret_cast<void**>(&dptr), cap * sizeof(T)));
+        }
+        if (!vec.empty())
+            SI_HIP(hipMemcpyAsync(dptr, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+        return SI_OK;
+    };
+    if (bm.size() > 0xFFFFFFFFull || pr.size() > 0xFFFFFFFFull || merge.size() > 0xFFFFFFFFull)
+        return fail(SI_ERR_RANGE, "too many work items");
+    SI_TRY(upload(bm, P->d_bm_items, P->items_cap_bm));
+    SI_TRY(upload(pr, P->d_pr_items, P->items_cap_pr));
+    SI_TRY(upload(merge, P->d_merge, P->merge_cap));
+    SI_HIP(hipStreamSynchronize(ctx->stream));
+    P->n_bm = uint32_t(bm.size());
+    P->n_pr = uint32_t(pr.size());
+    P->n_merge = uint32_t(merge.size());
+    P->tuned = t;
+    P->have_items = true;
+    if (P->gexec) {  // items changed: drop the captured sequence
+        (void)hipGraphExecDestroy(P->gexec);
+        (void)hipGraphDestroy(P->graph);
+        P->gexec = nullptr;
+        P->graph = nullptr;
+    }
+    return SI_OK;
+}
+
+// Enqueues the launch sequence of one batched count.
+static int enqueue(si_dataset* ds, const si_tuning& t, uint32_t* out) {
+    BatchPlan* P = ds->plan;
+    si_context* ctx = ds->ctx;
+    const uint32_t k = P->k;
+    hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, ctx->stream, P->d_queue);
+    if (P->n_bm) {
+        KernelScope ks(ctx, "batched_rows_bitmap");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(ctx->num_cus, P->n_bm));
+        hipLaunchKernelGGL((k_rows<BM_NT, BM_WORDS>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_bm_items, P->n_bm,
+                           P->d_queue + 0, P->d_selem, P->d_soff, P->d_perm, k, bitmap_words(ds), out);
+    }
+    if (P->n_pr) {
+        KernelScope ks(ctx, "batched_rows_probe");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * t.probe_wg_per_cu, P->n_pr));
+        hipLaunchKernelGGL((k_rows<PR_NT, PR_WORDS>), dim3(grid), dim3(PR_NT), 0, ctx->stream, P->d_pr_items, P->n_pr,
+                           P->d_queue + 1, P->d_selem, P->d_soff, P->d_perm, k, bitmap_words(ds), out);
+    }
+    if (P->n_merge) {
+        KernelScope ks(ctx, "batched_merge_pairs");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * 4, P->n_merge));
+        hipLaunchKernelGGL((k_merge_pairs<256, 16>), dim3(grid), dim3(256), 0, ctx->stream, P->d_merge, P->n_merge,
+                           ds->d_elems, ds->d_offsets, out);
+    }
+    if (P->kne < k && k > 1) {
+        KernelScope ks(ctx, "batched_zero_pairs");
+        hipLaunchKernelGGL(k_zero_pairs, dim3(std::min<uint32_t>(k, 4096)), dim3(256), 0, ctx->stream, P->d_perm, k,
+                           P->kne, out);
+    }
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+int batched_count(si_dataset* ds, const si_tuning* tuning, uint32_t* d_out) {
+    si_tuning t;
+    if (tuning) {
+        t = *tuning;
+        SI_TRY(check_tuning(t));
+    } else {
+        batched_default_tuning(ds, &t);
+    }
+    if (ds->k < 2) return SI_OK;
+    SI_TRY(build_plan(ds));
+    SI_TRY(build_items(ds, t));
+    BatchPlan* P = ds->plan;
+    si_context* ctx = ds->ctx;
+    // kernel timing records events between launches, which a graph cannot hold
+    if (!t.use_graph || ctx->kernel_timing) return enqueue(ds, t, d_out);
+    if (!P->gexec || P->g_out != d_out || !same_tuning(P->g_tuning, t)) {
+        if (P->gexec) {
+            (void)hipGraphExecDestroy(P->gexec);
+            (void)hipGraphDestroy(P->graph);
+            P->gexec = nullptr;
+            P->graph = nullptr;
+        }
+        SI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int st = enqueue(ds, t, d_out);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (st != SI_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return fail(SI_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        P->graph = g;
+        SI_HIP(hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0));
+        P->g_out = d_out;
+        P->g_tuning = t;
+    }
+    SI_HIP(hipGraphLaunch(P->gexec, ctx->stream));
+    return SI_OK;
+}
+
+int batched_autotune(si_dataset* ds, double budget_ms, si_tuning* best) {
+    si_context* ctx = ds->ctx;
+    si_tuning base;
+    batched_default_tuning(ds, &base);
+    base.use_graph = 0;
+    *best = base;
+    if (ds->k < 2) return SI_OK;
+    const uint64_t slots = uint64_t(ds->k) * (ds->k - 1) / 2;
+    void* out = nullptr;
+    SI_HIP(hipMalloc(&out, slots * 4));
+    hipEvent_t e0, e1;
+    SI_HIP(hipEventCreate(&e0));
+    SI_HIP(hipEventCreate(&e1));
+    double spent = 0.0, bestMs = 1e300;
+    auto measure = [&](const si_tuning& t, double& ms) -> int {
+        SI_TRY(batched_count(ds, &t, static_cast<uint32_t*>(out)));  // builds items, warms caches
+        SI_HIP(hipEventRecord(e0, ctx->stream));
+        SI_TRY(batched_count(ds, &t, static_cast<uint32_t*>(out)));
+        SI_HIP(hipEventRecord(e1, ctx->stream));
+        SI_HIP(hipEventSynchronize(e1));
+        float f = 0.f;
+        SI_HIP(hipEventElapsedTime(&f, e0, e1));
+        ms = f;
+        spent += 2.0 * f;
+        return SI_OK;
+    };
+    int st = SI_OK;
+    double ms = 0;
+    st = measure(base, ms);
+    if (st == SI_OK) bestMs = ms;
+    // coordinate descent over the dispatcher's knobs
+    const std::vector<std::pair<uint32_t si_tuning::*, std::vector<uint32_t>>> knobs = {
+        {&si_tuning::bitmap_min, {512, 1024, 2048, 4096}},
+        {&si_tuning::chunk_elems, {16384, 32768, 65536, 131072, 262144}},
+        {&si_tuning::chunk_elems_bitmap, {262144, 524288, 1u << 20, 1u << 21}},
+        {&si_tuning::probe_wg_per_cu, {2, 4, 6, 8}},
+        {&si_tuning::ratio_search, {8, 16, 32, 64}},
+    };
+    for (int pass = 0; pass < 2 && st == SI_OK; ++pass) {
+        for (auto& kv : knobs) {
+            for (uint32_t v : kv.second) {
+                if (st != SI_OK || spent > budget_ms) break;
+                si_tuning t = *best;
+                if (t.*(kv.first) == v) continue;
+                t.*(kv.first) = v;
+                st = measure(t, ms);
+                if (st == SI_OK && ms < bestMs) {
+                    bestMs = ms;
+                    *best = t;
+                }
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    best->use_graph = 1;
+    return st;
+}
+
+// ------------------------------------------------------------- edge lists --
+}  // namespace si_impl
+
+struct si_edge_plan {
+    si_dataset* ds = nullptr;
+    uint64_t m = 0;
+    si_tuning t{};
+    Item* d_bm = nullptr;
+    Item* d_pr = nullptr;
+    uint2* d_plist = nullptr;
+    uint4* d_merge = nullptr;
+    uint32_t* d_queue = nullptr;
+    uint64_t* d_partials = nullptr;
+    uint32_t n_bm = 0, n_pr = 0, n_merge = 0;
+    ~si_edge_plan() {
+        for (void* p : {(void*)d_bm, (void*)d_pr, (void*)d_plist, (void*)d_merge, (void*)d_queue, (void*)d_partials})
+            if (p) (void)hipFree(p);
+    }
+};
+
+namespace si_impl {
+
+constexpr int SUM_GRID = 1024;
+
+int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, const si_tuning* tuning,
+                     si_edge_plan** out) {
+    si_tuning t;
+    if (tuning) {
+        t = *tuning;
+        SI_TRY(check_tuning(t));
+    } else {
+        batched_default_tuning(ds, &t);
+    }
+    const uint32_t k = ds->k;
+    if (m > 0xFFFFFFFFull) return fail(SI_ERR_RANGE, "at most 2^32-1 edges per plan");
+    for (uint64_t e = 0; e < m; ++e)
+        if (u[e] >= k || v[e] >= k) return fail(SI_ERR_ARG, "edge endpoint out of range (edge " + std::to_string(e) + ")");
+    auto plan = std::make_unique<si_edge_plan>();
+    plan->ds = ds;
+    plan->m = m;
+    plan->t = t;
+    const auto& sz = ds->sizes;
+    // row = endpoint with the larger list (ties: the smaller id); counting sort by row
+    std::vector<uint32_t> rowOf(m), cnt(size_t(k) + 1, 0);
+    for (uint64_t e = 0; e < m; ++e) {
+        const uint32_t a = u[e], b = v[e];
+        const uint32_t r = (sz[a] > sz[b] || (sz[a] == sz[b] && a <= b)) ? a : b;
+        rowOf[e] = r;
+        cnt[r + 1]++;
+    }
+    for (uint32_t i = 0; i < k; ++i) cnt[i + 1] += cnt[i];
+    std::vector<uint2> plist(m);
+    {
+        std::vector<uint32_t> pos(cnt.begin(), cnt.end() - 1);
+        for (uint64_t e = 0; e < m; ++e) {
+            const uint32_t r = rowOf[e];
+            plist[pos[r]++] = make_uint2(u[e] == r ? v[e] : u[e], uint32_t(e));
+        }
+    }
+    const bool bmFits = bitmap_words(ds) <= uint32_t(BM_WORDS);
+    std::vector<Item> bm, pr;
+    std::vector<uint4> merge;
+    std::vector<uint2> plist2;  // final partner order (SEARCH/probe partners of BIG rows first-class)
+    plist2.reserve(m);
+    auto emit = [&](std::vector<Item>& vec, uint32_t row, uint32_t p0, uint32_t p1, uint32_t mode, uint64_t chunk) {
+        uint32_t p = p0;
+        while (p < p1) {
+            uint64_t acc = 0;
+            uint32_t e = p;
+            while (e < p1 && (e == p || acc + sz[plist2[e].x] <= chunk)) acc += sz[plist2[e++].x];
+            vec.push_back(Item{row, p, e, mode});
+            p = e;
+        }
+    };
+    for (uint32_t r = 0; r < k; ++r) {
+        if (cnt[r] == cnt[r + 1]) continue;
+        const uint32_t s = sz[r];
+        const uint32_t start = uint32_t(plist2.size());
+        if ((bmFits && (s >= t.bitmap_min || s > HASH_MAX)) || s <= HASH_MAX) {
+            for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) plist2.push_back(plist[q]);
+            const bool useBm = bmFits && (s >= t.bitmap_min || s > HASH_MAX);
+            if (useBm)
+                emit(bm, r, start, uint32_t(plist2.size()), MODE_BITMAP, t.chunk_elems_bitmap);
+            else
+                emit(pr, r, start, uint32_t(plist2.size()), MODE_HASH, t.chunk_elems);
+        } else {
+            for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) {
+                const uint2 pe = plist[q];
+                if (uint64_t(sz[pe.x]) * t.ratio_search > s)
+                    merge.push_back(make_uint4(r, pe.x, pe.y, 0));
+                else
+                    plist2.push_back(pe);
+            }
+            if (plist2.size() > start) emit(pr, r, start, uint32_t(plist2.size()), MODE_SEARCH, t.chunk_elems);
+        }
+    }
+    si_context* ctx = ds->ctx;
+    auto up = [&](auto& vec, auto*& dptr) -> int {
+        using T = typename std::decay<decltype(vec)>::type::value_type;
+        SI_HIP(hipMalloc(reinterpret_cast<void**>(&dptr), std::max<size_t>(vec.size(), 1) * sizeof(T)));
+        if (!vec.empty())
+            SI_HIP(hipMemcpyAsync(dptr, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+        return SI_OK;
+    };
+    SI_TRY(up(bm, plan->d_bm));
+    SI_TRY(up(pr, plan->d_pr));
+    SI_TRY(up(plist2, plan->d_plist));
+    SI_TRY(up(merge, plan->d_merge));
+    SI_HIP(hipMalloc(&plan->d_queue, 256));
+    SI_HIP(hipMemsetAsync(plan->d_queue, 0, 256, ctx->stream));
+    SI_HIP(hipMalloc(&plan->d_partials, SUM_GRID * 8));
+    SI_HIP(hipStreamSynchronize(ctx->stream));
+    plan->n_bm = uint32_t(bm.size());
+    plan->n_pr = uint32_t(pr.size());
+    plan->n_merge = uint32_t(merge.size());
+    *out = plan.release();
+    return SI_OK;
+}
+
+int edge_count(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
+    si_dataset* ds = P->ds;
+    si_context* ctx = ds->ctx;
+    hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, ctx->stream, P->d_queue);
+    if (P->n_bm) {
+        KernelScope ks(ctx, "edges_rows_bitmap");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(ctx->num_cus, P->n_bm));
+        hipLaunchKernelGGL((k_rows_list<BM_NT, BM_WORDS>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_bm, P->n_bm,
+                           P->d_queue + 0, ds->d_elems, ds->d_offsets, P->d_plist, bitmap_words(ds), counts);
+    }
+    if (P->n_pr) {
+        KernelScope ks(ctx, "edges_rows_probe");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * P->t.probe_wg_per_cu, P->n_pr));
+        hipLaunchKernelGGL((k_rows_list<PR_NT, PR_WORDS>), dim3(grid), dim3(PR_NT), 0, ctx->stream, P->d_pr, P->n_pr,
+                           P->d_queue + 1, ds->d_elems, ds->d_offsets, P->d_plist, bitmap_words(ds), counts);
+    }
+    if (P->n_merge) {
+        KernelScope ks(ctx, "edges_merge_pairs");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * 4, P->n_merge));
+        hipLaunchKernelGGL((k_merge_pairs<256, 16>), dim3(grid), dim3(256), 0, ctx->stream, P->d_merge, P->n_merge,
+                           ds->d_elems, ds->d_offsets, counts);
+    }
+    {
+        KernelScope ks(ctx, "edges_sum");
+        const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(SUM_GRID, (P->m + 255) / 256)));
+        hipLaunchKernelGGL((k_sum_counts<256>), dim3(grid), dim3(256), 0, ctx->stream, counts, P->m, P->d_partials,
+                           P->d_queue + 2, total);
+    }
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+void edge_plan_free(si_edge_plan* p) { delete p; }
+
+uint64_t plan_edges(si_edge_plan* p) { return p->m; }
+
+}  // namespace si_impl

@@ -1,0 +1,294 @@
+// Bitmap kernels for gfx950: bitmap_dynamic (build a set's bitmap, probe the
+// other set's elements) and bitmap_naive (bitmaps of all sets, AND + popcount
+// for every pair).
+//
+// Reference behaviour
+//   bitmap_dynamic.cu:240-276  SISI: constructBitmap of set 0 with atomicOr,
+//                              singleInstanceIntersect probes set 1, per-block
+//                              counts, thrust::reduce.
+//   bitmap_dynamic.cu:279-319  MISI: per block and row a, a private global
+//                              bitmap, one atomicAdd per hit, then the touched
+//                              words are cleared.
+//   bitmap_naive.cu:153-181    generateBitmaps (all k bitmaps), then for every
+//                              pair a grid-stride AND + __popcll.
+//
+// MI355X design:
+//  * Bitmaps have universe+1 bits (elements reach `universe`; the reference
+//    sizes them one bit short, bitmap_naive.cu:68).
+//  * bitmap_dynamic SISI builds the SMALLER set (fewer atomics) and probes the
+//    larger; counts fold through per-workgroup partials (no reduce kernel).
+//  * bitmap_dynamic runs keep the row bitmap in LDS when universe+1 <= 2^20
+//    bits (128 KiB of the 160 KiB LDS), else in a per-workgroup global region;
+//    each pair's hits are reduced with wave shuffles (one atomic per wave,
+//    not one per hit).
+//  * bitmap_naive computes all pairs as a tiled AND/popcount "GEMM": a
+//    workgroup owns a 64 x 64 block of pairs and streams both operand row
+//    blocks through LDS in chunks of 16 words, so every bitmap word loaded is
+//    reused 64 times (the reference re-reads both bitmaps for every pair).
+#include "si_device.h"
+#include "si_internal.h"
+
+namespace {
+
+using namespace si_dev;
+
+// ------------------------------------------------------- bitmap_dynamic --
+__global__ void k_bm_build(const uint32_t* __restrict__ S, uint64_t n, uint32_t* __restrict__ bm) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t x = S[i];
+        atomicOr(&bm[x >> 5], 1u << (x & 31));
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_bm_probe(const uint32_t* __restrict__ P, uint64_t n,
+                                                 const uint32_t* __restrict__ bm, uint64_t words,
+                                                 uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
+                                                 uint64_t* __restrict__ out64, uint32_t* __restrict__ out32) {
+    __shared__ uint64_t red[NT / WAVE];
+    uint64_t c = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * NT;
+    for (uint64_t i = uint64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
+        const uint32_t x = P[i];
+        const uint64_t w = x >> 5;
+        if (w < words) c += (bm[w] >> (x & 31)) & 1u;
+    }
+    grid_fold_u64<NT>(c, partials, done, out64, out32, red);
+}
+
+// MISI run: one workgroup per row a; bitmap of WORDS 32-bit words in LDS
+// (LDS_WORDS > 0) or in gbm + blockIdx * words (LDS_WORDS == 0).
+template <int NT, int LDS_WORDS, bool SELF>
+__global__ __launch_bounds__(NT) void k_bm_run(const uint32_t* __restrict__ base, const uint64_t* __restrict__ offsets,
+                                               RunArgs r, uint32_t* __restrict__ counts, uint32_t* __restrict__ gbm,
+                                               uint64_t words) {
+    __shared__ uint32_t lbm[LDS_WORDS > 0 ? LDS_WORDS : 1];
+    uint32_t* bm = LDS_WORDS > 0 ? lbm : gbm + uint64_t(blockIdx.x) * words;
+    for (uint64_t i = threadIdx.x; i < words; i += NT) bm[i] = 0;
+    __threadfence_block();
+    __syncthreads();
+    for (uint32_t a = r.aStart + blockIdx.x; a < r.aEnd; a += gridDim.x) {
+        const uint64_t oA = offsets[a], nA = offsets[a + 1] - oA;
+        if (nA == 0) continue;
+        for (uint64_t i = threadIdx.x; i < nA; i += NT) {
+            const uint32_t x = base[oA + i];
+            atomicOr(&bm[x >> 5], 1u << (x & 31));
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (uint32_t b = SELF ? a + 1 : r.bStart; b < r.bEnd; ++b) {
+            const uint64_t oB = offsets[b], nB = offsets[b + 1] - oB;
+            uint32_t c = 0;
+            for (uint64_t i = threadIdx.x; i < nB; i += NT) {
+                const uint32_t x = base[oB + i];
+                const uint64_t w = x >> 5;
+                if (w < words) c += (bm[w] >> (x & 31)) & 1u;
+            }
+            c = wave_sum_u32(c);
+            if (lane_id() == 0 && c) atomicAdd(&counts[run_index<SELF>(r, a, b)], c);
+        }
+        __syncthreads();
+        // clear the words set by row a
+        for (uint64_t i = threadIdx.x; i < nA; i += NT) bm[base[oA + i] >> 5] = 0;
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------- bitmap_naive --
+// Bitmaps of sets [s0, s0 + ns): row (s - s0) has W64 words of 64 bits.
+__global__ void k_bm_build_rows(const uint32_t* __restrict__ base, const uint64_t* __restrict__ offsets, uint32_t s0,
+                                uint32_t ns, uint64_t w64, unsigned long long* __restrict__ rows) {
+    for (uint32_t s = blockIdx.y; s < ns; s += gridDim.y) {
+        const uint64_t o = offsets[s0 + s], n = offsets[s0 + s + 1] - o;
+        unsigned long long* row = rows + uint64_t(s) * w64;
+        for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+            const uint32_t x = base[o + i];
+            atomicOr(&row[x >> 6], 1ull << (x & 63));
+        }
+    }
+}
+
+// AND + popcount over bit rows: C(i, j) = popcount(R_i & R_j) for i in row
+// block [i0, i0+ni) of RA and j in [j0, j0+nj) of RB.
+//   TRI: write counts[tri_index(n, i, j)] for j > i only (RA == RB).
+//   else dense: C[(i - i0) * ld + (j - j0)].
+template <bool TRI>
+__global__ __launch_bounds__(256) void k_popcount_pairs(const unsigned long long* __restrict__ RA, uint32_t ni,
+                                                        const unsigned long long* __restrict__ RB, uint32_t nj,
+                                                        uint64_t w64, uint32_t tilesJ, uint32_t* __restrict__ out,
+                                                        uint64_t ld) {
+    constexpr int T = 64, KC = 16;
+    __shared__ unsigned long long As[T][KC + 1];
+    __shared__ unsigned long long Bs[T][KC + 1];
+    uint32_t ti, tj;
+    if (TRI) {
+        tri_decode(blockIdx.x, tilesJ + 1, ti, tj);  // tiles ti <= tj encoded as ti < tj+1
+        tj -= 1;
+    } else {
+        ti = blockIdx.x / tilesJ;
+        tj = blockIdx.x % tilesJ;
+    }
+    const uint32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    uint32_t acc[4][4] = {};
+    const uint32_t lr = threadIdx.x >> 2, lc = (threadIdx.x & 3) * 4;  // loader: row, 4 words
+    for (uint64_t k0 = 0; k0 < w64; k0 += KC) {
+        {
+            const uint32_t gi = ti * T + lr, gj = tj * T + lr;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint64_t kk = k0 + lc + q;
+                As[lr][lc + q] = (gi < ni && kk < w64) ? RA[uint64_t(gi) * w64 + kk] : 0ull;
+                Bs[lr][lc + q] = (gj < nj && kk < w64) ? RB[uint64_t(gj) * w64 + kk] : 0ull;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+            unsigned long long a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = As[ty + 16 * r][kk];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] = Bs[tx + 16 * c][kk];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[r][c] += __popcll(a[r] & b[c]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t i = ti * T + ty + 16 * r;
+        if (i >= ni) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t j = tj * T + tx + 16 * c;
+            if (j >= nj) continue;
+            if (TRI) {
+                if (j > i) out[tri_index(ld, i, j)] = acc[r][c];
+            } else {
+                out[uint64_t(i) * ld + j] = acc[r][c];
+            }
+        }
+    }
+}
+
+constexpr int NT = 256;
+
+}  // namespace
+
+namespace si_impl {
+
+static uint64_t bitmap_words32(si_dataset* ds) { return (uint64_t(std::max(ds->max_elem, ds->universe)) >> 5) + 1; }
+
+int bitmap_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32) {
+    si_context* ctx = ds->ctx;
+    uint32_t s = a, q = b;  // build the smaller set, probe the larger
+    if (ds->sizes[s] > ds->sizes[q]) std::swap(s, q);
+    const uint64_t nS = ds->sizes[s], nQ = ds->sizes[q];
+    if (nS == 0 || nQ == 0) {
+        return zero_outputs(ctx, d_out, d_out32);
+    }
+    const uint64_t words = bitmap_words32(ds);
+    SI_TRY(ctx->ws_aux.ensure(words * 4));
+    uint32_t* bm = ctx->ws_aux.as<uint32_t>();
+    SI_HIP(hipMemsetAsync(bm, 0, words * 4, ctx->stream));
+    const uint64_t maxGrid = (p && p->blocks) ? p->blocks : uint64_t(ctx->num_cus) * 8;
+    {
+        KernelScope ks(ctx, "bitmap_sisi_build");
+        const uint64_t g = std::max<uint64_t>(1, std::min<uint64_t>((nS + NT - 1) / NT, maxGrid));
+        hipLaunchKernelGGL(k_bm_build, dim3(uint32_t(g)), dim3(NT), 0, ctx->stream, ds->d_elems + ds->offsets[s], nS, bm);
+        SI_HIP(hipGetLastError());
+    }
+    const uint64_t g = std::max<uint64_t>(1, std::min<uint64_t>((nQ + NT - 1) / NT, maxGrid));
+    void* ws = nullptr;
+    SI_TRY(small_workspace(ctx, 256 + g * 8, &ws));
+    KernelScope ks(ctx, "bitmap_sisi_probe");
+    hipLaunchKernelGGL((k_bm_probe<NT>), dim3(uint32_t(g)), dim3(NT), 0, ctx->stream, ds->d_elems + ds->offsets[q], nQ,
+                       bm, words, reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256), static_cast<uint32_t*>(ws),
+                       d_out, d_out32);
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+int bitmap_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts) {
+    si_context* ctx = ds->ctx;
+    const bool self = (A.id == B.id);
+    RunArgs r{A.start, A.end, A.id, B.start, B.end, B.id, n};
+    const uint64_t words = bitmap_words32(ds);
+    uint64_t grid = (p && p->blocks) ? p->blocks : uint64_t(ctx->num_cus) * 4;
+    grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, A.end - A.start));
+    KernelScope ks(ctx, "bitmap_run");
+    constexpr int SMALL = 8192, LARGE = 32768;  // 32 KiB / 128 KiB of LDS
+    if (words <= SMALL) {
+        if (self)
+            hipLaunchKernelGGL((k_bm_run<NT, SMALL, true>), dim3(uint32_t(grid)), dim3(NT), 0, ctx->stream, ds->d_elems,
+                               ds->d_offsets, r, d_counts, nullptr, words);
+        else
+            hipLaunchKernelGGL((k_bm_run<NT, SMALL, false>), dim3(uint32_t(grid)), dim3(NT), 0, ctx->stream, ds->d_elems,
+                               ds->d_offsets, r, d_counts, nullptr, words);
+    } else if (words <= LARGE) {
+        if (self)
+            hipLaunchKernelGGL((k_bm_run<NT, LARGE, true>), dim3(uint32_t(grid)), dim3(NT), 0, ctx->stream, ds->d_elems,
+                               ds->d_offsets, r, d_counts, nullptr, words);
+        else
+            hipLaunchKernelGGL((k_bm_run<NT, LARGE, false>), dim3(uint32_t(grid)), dim3(NT), 0, ctx->stream, ds->d_elems,
+                               ds->d_offsets, r, d_counts, nullptr, words);
+    } else {
+        const uint64_t budget = 4ull << 30;
+        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, budget / (words * 4)));
+        SI_TRY(ctx->ws_aux.ensure(grid * words * 4));
+        if (self)
+            hipLaunchKernelGGL((k_bm_run<NT, 0, true>), dim3(uint32_t(grid)), dim3(NT), 0, ctx->stream, ds->d_elems,
+                               ds->d_offsets, r, d_counts, ctx->ws_aux.as<uint32_t>(), words);
+        else
+            hipLaunchKernelGGL((k_bm_run<NT, 0, false>), dim3(uint32_t(grid)), dim3(NT), 0, ctx->stream, ds->d_elems,
+                               ds->d_offsets, r, d_counts, ctx->ws_aux.as<uint32_t>(), words);
+    }
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+// Bit rows of sets [s0, s0+ns) into `rows` (ns x w64 words), zeroed first.
+int build_bit_rows(si_dataset* ds, uint32_t s0, uint32_t ns, uint64_t w64, unsigned long long* rows) {
+    si_context* ctx = ds->ctx;
+    SI_HIP(hipMemsetAsync(rows, 0, uint64_t(ns) * w64 * 8, ctx->stream));
+    if (ns == 0) return SI_OK;
+    KernelScope ks(ctx, "bitmap_build_rows");
+    dim3 grid(8, std::min<uint32_t>(ns, 65535));
+    hipLaunchKernelGGL(k_bm_build_rows, grid, dim3(NT), 0, ctx->stream, ds->d_elems, ds->d_offsets, s0, ns, w64, rows);
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+int popcount_pairs(si_context* ctx, const unsigned long long* RA, uint32_t ni, const unsigned long long* RB, uint32_t nj,
+                   uint64_t w64, bool tri, uint32_t* out, uint64_t ld) {
+    const uint32_t ti = (ni + 63) / 64, tj = (nj + 63) / 64;
+    const uint64_t blocks = tri ? uint64_t(tj) * (tj + 1) / 2 : uint64_t(ti) * tj;
+    if (blocks == 0) return SI_OK;
+    if (blocks > 0x7FFFFFFFull) return fail(SI_ERR_RANGE, "too many pair tiles");
+    KernelScope ks(ctx, "popcount_pairs");
+    if (tri)
+        hipLaunchKernelGGL((k_popcount_pairs<true>), dim3(uint32_t(blocks)), dim3(256), 0, ctx->stream, RA, ni, RB, nj, w64,
+                           tj, out, ld);
+    else
+        hipLaunchKernelGGL((k_popcount_pairs<false>), dim3(uint32_t(blocks)), dim3(256), 0, ctx->stream, RA, ni, RB, nj,
+                           w64, tj, out, ld);
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+// bitmap_naive: all pairs of the collection, counts[triangular_index(k, a, b)].
+int bitmap_all_pairs(si_dataset* ds, const si_params*, uint32_t* d_counts) {
+    si_context* ctx = ds->ctx;
+    const uint64_t w64 = (uint64_t(std::max(ds->max_elem, ds->universe)) >> 6) + 1;
+    const uint64_t bytes = uint64_t(ds->k) * w64 * 8;
+    SI_TRY(ctx->ws_aux.ensure(std::max<uint64_t>(bytes, 64)));
+    auto* rows = ctx->ws_aux.as<unsigned long long>();
+    SI_TRY(build_bit_rows(ds, 0, ds->k, w64, rows));
+    return popcount_pairs(ctx, rows, ds->k, rows, ds->k, w64, true, d_counts, ds->k);
+}
+
+}  // namespace si_impl

@@ -29,217 +29,11 @@
 //    slot; a second kernel places every slot at its prefix-sum offset.
 #include "si_device.h"
 #include "si_internal.h"
+#include "si_merge.h"
 
 namespace {
 
 using namespace si_dev;
-
-template <int NT, int VT>
-struct MergeCfg {
-    static constexpr int T = NT * VT;          // merged elements per tile
-    static constexpr uint32_t M = T - 1;       // circular window mask
-    static constexpr int CPT = (T / 4 + 4 + NT - 1) / NT;  // 16-byte chunks per thread per refill
-    static_assert((T & (T - 1)) == 0, "tile must be a power of two");
-};
-
-template <int NT, int VT>
-struct MergeSmem {
-    uint32_t wa[NT * VT];
-    uint32_t wb[NT * VT];
-    uint32_t ends[NT];
-    uint32_t wave_tot[NT / WAVE];
-    uint64_t red[NT / WAVE];
-    uint64_t seg[4];
-};
-
-// Loads of one window refill, held in registers across the tile's merge.
-template <int NT, int VT>
-struct Refill {
-    using C = MergeCfg<NT, VT>;
-    uint4 v[C::CPT];
-    uint64_t cA0, cB0;     // first chunk index of the A / B ranges
-    uint32_t nCA, nCB;     // chunk counts
-    uint64_t gA0, gA1, gB0, gB1;  // element ranges [g0, g1) to write
-
-    __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t a0, uint64_t a1, uint64_t b0,
-                                          uint64_t b1) {
-        gA0 = a0;
-        gA1 = a1;
-        gB0 = b0;
-        gB1 = b1;
-        cA0 = a0 >> 2;
-        nCA = a1 > a0 ? uint32_t(((a1 + 3) >> 2) - cA0) : 0u;
-        cB0 = b0 >> 2;
-        nCB = b1 > b0 ? uint32_t(((b1 + 3) >> 2) - cB0) : 0u;
-        const uint4* __restrict__ b4 = reinterpret_cast<const uint4*>(base);
-#pragma unroll
-        for (int j = 0; j < C::CPT; ++j) {
-            const uint32_t c = threadIdx.x + uint32_t(j) * NT;
-            if (c < nCA)
-                v[j] = b4[cA0 + c];
-            else if (c < nCA + nCB)
-                v[j] = b4[cB0 + (c - nCA)];
-        }
-    }
-
-    __device__ __forceinline__ static void put(uint32_t* w, const uint4& x, uint64_t g, uint64_t lo, uint64_t hi) {
-        if (g >= lo && g + 4 <= hi) {
-            *reinterpret_cast<uint4*>(&w[uint32_t(g) & C::M]) = x;
-        } else {
-            if (g + 0 >= lo && g + 0 < hi) w[uint32_t(g + 0) & C::M] = x.x;
-            if (g + 1 >= lo && g + 1 < hi) w[uint32_t(g + 1) & C::M] = x.y;
-            if (g + 2 >= lo && g + 2 < hi) w[uint32_t(g + 2) & C::M] = x.z;
-            if (g + 3 >= lo && g + 3 < hi) w[uint32_t(g + 3) & C::M] = x.w;
-        }
-    }
-
-    __device__ __forceinline__ void commit(MergeSmem<NT, VT>& sm) const {
-#pragma unroll
-        for (int j = 0; j < C::CPT; ++j) {
-            const uint32_t c = threadIdx.x + uint32_t(j) * NT;
-            if (c < nCA)
-                put(sm.wa, v[j], (cA0 + c) * 4, gA0, gA1);
-            else if (c < nCA + nCB)
-                put(sm.wb, v[j], (cB0 + (c - nCA)) * 4, gB0, gB1);
-        }
-    }
-};
-
-// Streams the merge path from (ia, ib) to (iaEnd, ibEnd) of sets A (global
-// element offset oA, size nA) and B (oB, nB) through the LDS windows.
-// Returns this thread's match count.  With MAT, matches are written in order
-// to stage[0 ..) (the segment's staging slot).
-template <int NT, int VT, bool MAT>
-__device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
-                                  uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
-                                  MergeSmem<NT, VT>& sm, uint32_t* __restrict__ stage) {
-    using C = MergeCfg<NT, VT>;
-    constexpr uint32_t T = C::T;
-    const uint32_t tid = threadIdx.x;
-    uint64_t myCount = 0;
-    uint64_t written = 0;  // MAT: matches already staged by this workgroup
-
-    uint64_t d = ia + ib;
-    const uint64_t dEnd = iaEnd + ibEnd;
-    if (d >= dEnd) return 0;
-
-    // initial fill: A[ia, ia+T) and B[ib, ib+T), clipped to the sets
-    {
-        Refill<NT, VT> r;
-        const uint64_t a1 = (ia + T < nA ? ia + T : nA);
-        r.issue(base, oA + ia, oA + a1, 0, 0);
-        r.commit(sm);
-        const uint64_t b1 = (ib + T < nB ? ib + T : nB);
-        r.issue(base, 0, 0, oB + ib, oB + b1);
-        r.commit(sm);
-    }
-    __syncthreads();
-
-    while (d < dEnd) {
-        const uint32_t Tc = uint32_t((dEnd - d) < T ? (dEnd - d) : T);
-        const uint32_t nAw = uint32_t((nA - ia) < T ? (nA - ia) : T);
-        const uint32_t nBw = uint32_t((nB - ib) < T ? (nB - ib) : T);
-        const uint32_t gA = uint32_t(oA + ia);  // slot base (mod T) of the windows
-        const uint32_t gB = uint32_t(oB + ib);
-        auto WA = [&](uint32_t i) { return sm.wa[(gA + i) & C::M]; };
-        auto WB = [&](uint32_t i) { return sm.wb[(gB + i) & C::M]; };
-
-        // 1. split at the end of this thread's diagonal range
-        const uint32_t dd = ((tid + 1) * VT < Tc) ? (tid + 1) * VT : Tc;
-        {
-            uint32_t lo = dd > nBw ? dd - nBw : 0u;
-            uint32_t hi = dd < nAw ? dd : nAw;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (WA(mid) <= WB(dd - 1 - mid))
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            sm.ends[tid] = lo;
-        }
-        __syncthreads();
-        const uint32_t ds = (tid * VT < Tc) ? tid * VT : Tc;
-        const uint32_t x0 = tid ? sm.ends[tid - 1] : 0u;
-        const uint32_t x1 = sm.ends[tid];
-        const uint32_t y0 = ds - x0;
-        const uint32_t y1 = dd - x1;
-        const uint32_t ca = sm.ends[NT - 1];
-        const uint32_t cb = Tc - ca;
-
-        // 2. fetch the elements that replace the consumed ones (registers)
-        Refill<NT, VT> r;
-        {
-            const uint64_t a0 = ia + T < nA ? ia + T : nA;
-            const uint64_t a1 = ia + T + ca < nA ? ia + T + ca : nA;
-            const uint64_t b0 = ib + T < nB ? ib + T : nB;
-            const uint64_t b1 = ib + T + cb < nB ? ib + T + cb : nB;
-            r.issue(base, oA + a0, oA + a1, oB + b0, oB + b1);
-        }
-
-        // 3. merge this thread's VT steps; A is taken on ties and matched
-        //    against the current B head (which may lie past y1).
-        uint32_t x = x0, y = y0;
-        uint32_t av = x < x1 ? WA(x) : 0u;
-        uint32_t bv = y < nBw ? WB(y) : 0u;
-        const uint32_t steps = (x1 - x0) + (y1 - y0);
-        uint32_t cnt = 0;
-        uint32_t mval[MAT ? VT : 1];
-        uint32_t mbits = 0;
-#pragma unroll
-        for (int k = 0; k < VT; ++k) {
-            if (uint32_t(k) < steps) {
-                const bool takeA = (x < x1) && (y >= y1 || av <= bv);
-                const bool hit = takeA && (y < nBw) && (av == bv);
-                cnt += hit;
-                if (MAT) {
-                    mval[k] = av;
-                    mbits |= uint32_t(hit) << k;
-                }
-                if (takeA) {
-                    ++x;
-                    if (x < x1) av = WA(x);
-                } else {
-                    ++y;
-                    if (y < nBw) bv = WB(y);
-                }
-            }
-        }
-        myCount += cnt;
-
-        uint32_t excl = 0, wtot = 0;
-        if (MAT) {
-            const uint32_t incl = wave_inclusive_scan_u32(cnt);
-            excl = incl - cnt;
-            if (lane_id() == WAVE - 1) sm.wave_tot[tid / WAVE] = incl;
-        }
-        __syncthreads();  // all window reads of this tile are done
-
-        // 4. commit the refill; with MAT, stage this tile's matches
-        r.commit(sm);
-        if (MAT) {
-            const uint32_t w = tid / WAVE;
-            uint32_t before = 0;
-#pragma unroll
-            for (int i = 0; i < NT / WAVE; ++i) {
-                const uint32_t t = sm.wave_tot[i];
-                if (i < int(w)) before += t;
-                wtot += t;
-            }
-            uint64_t pos = written + before + excl;
-#pragma unroll
-            for (int k = 0; k < VT; ++k) {
-                if (mbits & (1u << k)) stage[pos++] = mval[k];
-            }
-            written += wtot;
-        }
-        ia += ca;
-        ib += cb;
-        d += Tc;
-        __syncthreads();
-    }
-    return myCount;
-}
 
 // ---------------------------------------------------------------- SISI ----
 // One workgroup per merge-path segment of the pair (A, B).
@@ -250,8 +44,8 @@ template <int NT, int VT, bool MAT>
 __global__ __launch_bounds__(NT) void k_sisi_merge(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
                                                    uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
                                                    uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
-                                                   uint64_t* __restrict__ out, uint32_t* __restrict__ stage,
-                                                   uint64_t stage_cap) {
+                                                   uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
+                                                   uint32_t* __restrict__ stage, uint64_t stage_cap) {
     __shared__ MergeSmem<NT, VT> sm;
     const uint32_t s = blockIdx.x;
     const uint64_t n = nA + nB;
@@ -288,7 +82,8 @@ __global__ __launch_bounds__(NT) void k_sisi_merge(const uint32_t* __restrict__ 
     for (uint32_t i = threadIdx.x; i < nseg; i += NT) acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     acc = block_sum_u64<NT>(acc, sm.red);
     if (threadIdx.x == 0) {
-        *out = acc;
+        if (out) *out = acc;
+        if (out32) *out32 = uint32_t(acc);
         *done = 0;
     }
 }
@@ -363,7 +158,7 @@ static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, uint3
     if (nseg == 0) nseg = 1;
 }
 
-int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out) {
+int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32) {
     si_context* ctx = ds->ctx;
     const uint64_t oA = ds->offsets[a], nA = ds->sizes[a];
     const uint64_t oB = ds->offsets[b], nB = ds->sizes[b];
@@ -374,13 +169,10 @@ int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, ui
     SI_TRY(small_workspace(ctx, 256 + uint64_t(nseg) * 8, &ws));
     uint32_t* done = static_cast<uint32_t*>(ws);
     uint64_t* partials = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256);
-    if (nA == 0 || nB == 0) {
-        SI_HIP(hipMemsetAsync(d_out, 0, 8, ctx->stream));
-        return SI_OK;
-    }
+    if (nA == 0 || nB == 0) return zero_outputs(ctx, d_out, d_out32);
     KernelScope ks(ctx, "ip_sisi_count");
     hipLaunchKernelGGL((k_sisi_merge<SISI_NT, SISI_VT, false>), dim3(nseg), dim3(SISI_NT), 0, ctx->stream,
-                       ds->d_elems, oA, nA, oB, nB, seg_len, nseg, partials, done, d_out, nullptr, uint64_t(0));
+                       ds->d_elems, oA, nA, oB, nB, seg_len, nseg, partials, done, d_out, d_out32, nullptr, uint64_t(0));
     SI_HIP(hipGetLastError());
     return SI_OK;
 }
@@ -406,7 +198,7 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
     {
         KernelScope ks(ctx, "ip_sisi_materialize");
         hipLaunchKernelGGL((k_sisi_merge<SISI_NT, SISI_VT, true>), dim3(nseg), dim3(SISI_NT), 0, ctx->stream,
-                           ds->d_elems, oA, nA, oB, nB, seg_len, nseg, partials, done, d_count, stage, cap);
+                           ds->d_elems, oA, nA, oB, nB, seg_len, nseg, partials, done, d_count, nullptr, stage, cap);
         SI_HIP(hipGetLastError());
     }
     {

@@ -40,7 +40,10 @@ int si_pair_count_async(si_dataset* ds, uint32_t a, uint32_t b, int32_t techniqu
     if (!out || offset + 8 > out->bytes || (offset & 7)) return fail(SI_ERR_ARG, "bad output buffer/offset");
     uint64_t* d = reinterpret_cast<uint64_t*>(static_cast<char*>(out->ptr) + offset);
     switch (technique) {
-        case SI_TECH_IP: return ip_pair_count(ds, a, b, params, d);
+        case SI_TECH_IP: return ip_pair_count(ds, a, b, params, d, nullptr);
+        case SI_TECH_OBS: return obs_pair_count(ds, a, b, params, d, nullptr);
+        case SI_TECH_HI: return hi_pair_count(ds, a, b, params, d, nullptr);
+        case SI_TECH_BITMAP_DYNAMIC: return bitmap_pair_count(ds, a, b, params, d, nullptr);
         default: return not_available("si_pair_count");
     }
 }
@@ -101,8 +104,34 @@ int si_run_count_async(si_dataset* ds, int32_t technique, const si_params* param
     si_context* ctx = ds->ctx;
     SI_HIP(hipMemsetAsync(counts->ptr, 0, slots * 4, ctx->stream));
     uint32_t* d = static_cast<uint32_t*>(counts->ptr);
+    // A run of a few pairs (SISI datasets are k = 2) is spread over the whole
+    // GPU pair by pair with the single-pair kernels.
+    const bool self = A.id == B.id;
+    const uint64_t la = A.end - A.start, lb = B.end - B.start;
+    const uint64_t npairs = self ? la * (la ? la - 1 : 0) / 2 : la * lb;
+    if (npairs <= 64 && technique != SI_TECH_BITMAP_NAIVE && technique != SI_TECH_MM) {
+        for (uint32_t i = A.start; i < A.end; ++i) {
+            for (uint32_t j = self ? i + 1 : B.start; j < B.end; ++j) {
+                const uint64_t li = uint64_t(i) - uint64_t(A.id) * num_sets, lj = uint64_t(j) - uint64_t(B.id) * num_sets;
+                const uint64_t n = num_sets;
+                const uint64_t idx = self ? (n * (n - 1) / 2) - (n - li) * ((n - li) - 1) / 2 + lj - li - 1 : li * n + lj;
+                uint32_t* slot = d + idx;
+                switch (technique) {
+                    case SI_TECH_IP: SI_TRY(ip_pair_count(ds, i, j, params, nullptr, slot)); break;
+                    case SI_TECH_OBS: SI_TRY(obs_pair_count(ds, i, j, params, nullptr, slot)); break;
+                    case SI_TECH_HI: SI_TRY(hi_pair_count(ds, i, j, params, nullptr, slot)); break;
+                    case SI_TECH_BITMAP_DYNAMIC: SI_TRY(bitmap_pair_count(ds, i, j, params, nullptr, slot)); break;
+                    default: return not_available("si_run_count");
+                }
+            }
+        }
+        return SI_OK;
+    }
     switch (technique) {
         case SI_TECH_IP: return ip_run(ds, params, A, B, num_sets, d);
+        case SI_TECH_OBS: return obs_run(ds, params, A, B, num_sets, d);
+        case SI_TECH_HI: return hi_run(ds, params, A, B, num_sets, d);
+        case SI_TECH_BITMAP_DYNAMIC: return bitmap_run(ds, params, A, B, num_sets, d);
         default: return not_available("si_run_count");
     }
 }
@@ -120,26 +149,111 @@ int si_run_count(si_dataset* ds, int32_t technique, const si_params* params, si_
     return st;
 }
 
-// ---- entry points completed in later files (declared in the public ABI) ----
-int si_all_pairs_bitmap_async(si_dataset* ds, const si_params*, si_buffer*) {
-    (void)ds;
-    return not_available("si_all_pairs_bitmap");
+int si_all_pairs_bitmap_async(si_dataset* ds, const si_params* params, si_buffer* counts) {
+    if (!ds || !counts) return fail(SI_ERR_ARG, "null argument");
+    const uint64_t slots = uint64_t(ds->k) * (ds->k > 0 ? ds->k - 1 : 0) / 2;
+    if (counts->bytes < slots * 4) return fail(SI_ERR_CAPACITY, "counts buffer smaller than k(k-1)/2 entries");
+    return bitmap_all_pairs(ds, params, static_cast<uint32_t*>(counts->ptr));
 }
-int si_all_pairs_bitmap(si_dataset* ds, const si_params*, uint32_t*) {
-    (void)ds;
-    return not_available("si_all_pairs_bitmap");
+
+int si_all_pairs_bitmap(si_dataset* ds, const si_params* params, uint32_t* counts_host) {
+    if (!ds || !counts_host) return fail(SI_ERR_ARG, "null argument");
+    const uint64_t slots = uint64_t(ds->k) * (ds->k > 0 ? ds->k - 1 : 0) / 2;
+    si_buffer* d = nullptr;
+    SI_TRY(si_buffer_alloc(ds->ctx, std::max<uint64_t>(slots, 1) * 4, &d));
+    int st = si_all_pairs_bitmap_async(ds, params, d);
+    if (st == SI_OK) st = si_buffer_download(d, 0, counts_host, slots * 4);
+    si_buffer_free(d);
+    return st;
 }
-int si_mm_counts_async(si_dataset*, si_tile, si_tile, int32_t, uint32_t, si_buffer*) { return not_available("si_mm_counts"); }
-int si_mm_counts(si_dataset*, si_tile, si_tile, int32_t, uint32_t, uint32_t*) { return not_available("si_mm_counts"); }
-int si_batched_default_tuning(si_dataset*, si_tuning*) { return not_available("si_batched_default_tuning"); }
-int si_batched_count(si_dataset*, const si_tuning*, uint32_t*) { return not_available("si_batched_count"); }
-int si_batched_count_async(si_dataset*, const si_tuning*, si_buffer*) { return not_available("si_batched_count"); }
-int si_batched_autotune(si_dataset*, double, si_tuning*) { return not_available("si_batched_autotune"); }
-int si_edge_count(si_dataset*, const uint32_t*, const uint32_t*, uint64_t, const si_tuning*, uint32_t*, uint64_t*) {
-    return not_available("si_edge_count");
+
+int si_mm_counts_async(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint32_t ld, si_buffer* counts) {
+    if (!ds || !counts) return fail(SI_ERR_ARG, "null argument");
+    SI_TRY(check_tile(ds, A));
+    SI_TRY(check_tile(ds, B));
+    if (variant < SI_MM_I8 || variant > SI_MM_POPCOUNT) return fail(SI_ERR_ARG, "unknown mm variant");
+    const uint64_t ni = A.end - A.start, nj = B.end - B.start;
+    if (ld < nj) return fail(SI_ERR_ARG, "ld smaller than B.len");
+    const uint64_t need = ni ? ((ni - 1) * ld + nj) : 0;
+    if (counts->bytes < need * 4) return fail(SI_ERR_CAPACITY, "counts buffer smaller than the tile matrix");
+    return mm_tile_counts(ds, A, B, variant, ld, static_cast<uint32_t*>(counts->ptr));
 }
-int si_edge_count_async(si_dataset*, si_buffer*, si_buffer*, uint64_t, const si_tuning*, si_buffer*, si_buffer*) {
-    return not_available("si_edge_count");
+
+int si_mm_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint32_t ld, uint32_t* counts_host) {
+    if (!ds || !counts_host) return fail(SI_ERR_ARG, "null argument");
+    const uint64_t ni = A.end > A.start ? A.end - A.start : 0, nj = B.end > B.start ? B.end - B.start : 0;
+    const uint64_t need = ni ? ((ni - 1) * uint64_t(ld) + nj) : 0;
+    si_buffer* d = nullptr;
+    SI_TRY(si_buffer_alloc(ds->ctx, std::max<uint64_t>(need, 1) * 4, &d));
+    int st = si_mm_counts_async(ds, A, B, variant, ld, d);
+    if (st == SI_OK) st = si_buffer_download(d, 0, counts_host, need * 4);
+    si_buffer_free(d);
+    return st;
+}
+
+int si_batched_default_tuning(si_dataset* ds, si_tuning* out) {
+    if (!ds || !out) return fail(SI_ERR_ARG, "null argument");
+    return batched_default_tuning(ds, out);
+}
+
+int si_batched_count_async(si_dataset* ds, const si_tuning* tuning, si_buffer* counts) {
+    if (!ds || !counts) return fail(SI_ERR_ARG, "null argument");
+    const uint64_t slots = uint64_t(ds->k) * (ds->k > 0 ? ds->k - 1 : 0) / 2;
+    if (counts->bytes < slots * 4) return fail(SI_ERR_CAPACITY, "counts buffer smaller than k(k-1)/2 entries");
+    return batched_count(ds, tuning, static_cast<uint32_t*>(counts->ptr));
+}
+
+int si_batched_count(si_dataset* ds, const si_tuning* tuning, uint32_t* counts_host) {
+    if (!ds || !counts_host) return fail(SI_ERR_ARG, "null argument");
+    const uint64_t slots = uint64_t(ds->k) * (ds->k > 0 ? ds->k - 1 : 0) / 2;
+    si_buffer* d = nullptr;
+    SI_TRY(si_buffer_alloc(ds->ctx, std::max<uint64_t>(slots, 1) * 4, &d));
+    int st = si_batched_count_async(ds, tuning, d);
+    if (st == SI_OK) st = si_buffer_download(d, 0, counts_host, slots * 4);
+    si_buffer_free(d);
+    return st;
+}
+
+int si_batched_autotune(si_dataset* ds, double budget_ms, si_tuning* out) {
+    if (!ds || !out) return fail(SI_ERR_ARG, "null argument");
+    return batched_autotune(ds, budget_ms, out);
+}
+
+int si_edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, const si_tuning* tuning,
+                        si_edge_plan** out) {
+    if (!ds || !out || (m && (!u || !v))) return fail(SI_ERR_ARG, "null argument");
+    *out = nullptr;
+    return edge_plan_create(ds, u, v, m, tuning, out);
+}
+
+int si_edge_plan_free(si_edge_plan* plan) {
+    if (plan) edge_plan_free(plan);
+    return SI_OK;
+}
+
+int si_edge_count_async(si_edge_plan* plan, si_buffer* counts, si_buffer* total_out) {
+    if (!plan || !counts || !total_out) return fail(SI_ERR_ARG, "null argument");
+    if (total_out->bytes < 8) return fail(SI_ERR_CAPACITY, "total buffer smaller than 8 bytes");
+    if (counts->bytes < plan_edges(plan) * 4) return fail(SI_ERR_CAPACITY, "counts buffer smaller than m entries");
+    return edge_count(plan, static_cast<uint32_t*>(counts->ptr), static_cast<uint64_t*>(total_out->ptr));
+}
+
+int si_edge_count(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, const si_tuning* tuning,
+                  uint32_t* counts_host, uint64_t* total) {
+    if (!ds || (m && (!u || !v || !counts_host)) || !total) return fail(SI_ERR_ARG, "null argument");
+    si_edge_plan* plan = nullptr;
+    SI_TRY(si_edge_plan_create(ds, u, v, m, tuning, &plan));
+    si_buffer* dc = nullptr;
+    si_buffer* dt = nullptr;
+    int st = si_buffer_alloc(ds->ctx, std::max<uint64_t>(m, 1) * 4, &dc);
+    if (st == SI_OK) st = si_buffer_alloc(ds->ctx, 8, &dt);
+    if (st == SI_OK) st = si_edge_count_async(plan, dc, dt);
+    if (st == SI_OK) st = si_buffer_download(dc, 0, counts_host, m * 4);
+    if (st == SI_OK) st = si_buffer_download(dt, 0, total, 8);
+    si_buffer_free(dc);
+    si_buffer_free(dt);
+    si_edge_plan_free(plan);
+    return st;
 }
 
 }  // extern "C"

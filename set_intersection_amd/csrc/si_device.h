@@ -52,6 +52,36 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* scratch)
     return t;
 }
 
+// Grid-wide sum of one uint64 per thread without atomics on the result:
+// every workgroup stores its partial, takes a ticket, and the last one folds
+// the partials into *out64 (and *out32 when given) and resets the ticket to
+// zero, so the counter needs no clearing between launches.
+template <int NT>
+__device__ __forceinline__ void grid_fold_u64(uint64_t v, uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
+                                              uint64_t* __restrict__ out64, uint32_t* __restrict__ out32,
+                                              uint64_t* scratch) {
+    __shared__ uint32_t isLast;
+    const uint64_t total = block_sum_u64<NT>(v, scratch);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = total;
+        __threadfence();
+        const uint32_t ticket = atomicAdd(done, 1u);
+        isLast = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!isLast) return;
+    __threadfence();
+    uint64_t acc = 0;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += NT)
+        acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc = block_sum_u64<NT>(acc, scratch);
+    if (threadIdx.x == 0) {
+        if (out64) *out64 = acc;
+        if (out32) *out32 = uint32_t(acc);
+        *done = 0;
+    }
+}
+
 // ------------------------------------------------------------- co-rank search
 // Merge order used everywhere: A[x] precedes B[y] iff A[x] <= B[y] (ties put
 // the A element first).  corank(d) = number of A elements among the first d
@@ -114,6 +144,20 @@ __device__ __forceinline__ uint64_t corank_serial(const uint32_t* __restrict__ A
 // triangular_index of include/util.hpp:68-70 in 64-bit arithmetic.
 __device__ __forceinline__ uint64_t tri_index(uint64_t n, uint64_t i, uint64_t j) {
     return (n * (n - 1) / 2) - (n - i) * ((n - i) - 1) / 2 + j - i - 1;
+}
+
+// One GPU invocation ("run") of the reference drivers: tile A x tile B with
+// the output layout of the run's counts buffer (ip.cu:145, helpers.cuh:16-22).
+struct RunArgs {
+    uint32_t aStart, aEnd, aId;
+    uint32_t bStart, bEnd, bId;
+    uint32_t n;  // numOfSets of the layout
+};
+
+template <bool SELF>
+__device__ __forceinline__ uint64_t run_index(const RunArgs& r, uint32_t a, uint32_t b) {
+    const uint64_t ia = uint64_t(a) - uint64_t(r.aId) * r.n, ib = uint64_t(b) - uint64_t(r.bId) * r.n;
+    return SELF ? tri_index(r.n, ia, ib) : ia * r.n + ib;
 }
 
 // Inverse of the row-major strict-upper-triangle enumeration of an L x L

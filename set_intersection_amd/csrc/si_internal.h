@@ -40,6 +40,9 @@ struct DevMem {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
+struct BatchPlan;
+void free_batch_plan(BatchPlan* p);
+
 }  // namespace si_impl
 
 struct si_context {
@@ -88,6 +91,7 @@ struct si_dataset {
     std::vector<uint64_t> offsets;  // k + 1
     uint32_t max_size = 0;
     uint32_t max_elem = 0;          // largest element of the collection
+    si_impl::BatchPlan* plan = nullptr;  // batched dispatcher state (built on first use)
 };
 
 namespace si_impl {
@@ -107,11 +111,39 @@ int check_pair(si_dataset* ds, uint32_t a, uint32_t b);
 // that use it leave it zeroed again when they finish.
 int small_workspace(si_context* ctx, uint64_t bytes, void** out);
 
+// Writes 0 to whichever of the two result slots is given (a pair with an
+// empty set).
+int zero_outputs(si_context* ctx, uint64_t* d_out64, uint32_t* d_out32);
+
 // ----------------------------------------------------- technique launchers --
 // SISI merge path (k_merge.hip)
-int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out);
+int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);
 int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out, uint64_t* d_count);
 // MISI runs
 int ip_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
+// optimized binary search (k_obs.hip)
+int obs_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);
+int obs_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
+// hash intersect (k_hash.hip)
+int hi_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);
+int hi_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
+// bitmaps (k_bitmap.hip)
+int bitmap_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);
+int bitmap_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
+int build_bit_rows(si_dataset* ds, uint32_t s0, uint32_t ns, uint64_t w64, unsigned long long* rows);
+int popcount_pairs(si_context* ctx, const unsigned long long* RA, uint32_t ni, const unsigned long long* RB, uint32_t nj,
+                   uint64_t w64, bool tri, uint32_t* out, uint64_t ld);
+int bitmap_all_pairs(si_dataset* ds, const si_params* p, uint32_t* d_counts);
+// bitmap GEMM (k_mm.hip)
+int mm_tile_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint64_t ld, uint32_t* d_C);
+// batched dispatcher (k_batched.hip)
+int batched_default_tuning(si_dataset* ds, si_tuning* t);
+int batched_count(si_dataset* ds, const si_tuning* tuning, uint32_t* d_out);
+int batched_autotune(si_dataset* ds, double budget_ms, si_tuning* best);
+int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, const si_tuning* tuning,
+                     si_edge_plan** out);
+int edge_count(si_edge_plan* plan, uint32_t* d_counts, uint64_t* d_total);
+void edge_plan_free(si_edge_plan* plan);
+uint64_t plan_edges(si_edge_plan* plan);
 
 }  // namespace si_impl

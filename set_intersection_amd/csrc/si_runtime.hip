@@ -99,6 +99,12 @@ int small_workspace(si_context* ctx, uint64_t bytes, void** out) {
     return SI_OK;
 }
 
+int zero_outputs(si_context* ctx, uint64_t* d_out64, uint32_t* d_out32) {
+    if (d_out64) SI_HIP(hipMemsetAsync(d_out64, 0, 8, ctx->stream));
+    if (d_out32) SI_HIP(hipMemsetAsync(d_out32, 0, 4, ctx->stream));
+    return SI_OK;
+}
+
 }  // namespace si_impl
 
 using namespace si_impl;
@@ -391,6 +397,7 @@ int si_dataset_upload(si_context* ctx, uint32_t k, uint32_t universe, uint64_t t
 int si_dataset_free(si_dataset* ds) {
     if (!ds) return SI_OK;
     (void)hipStreamSynchronize(ds->ctx->stream);
+    if (ds->plan) free_batch_plan(ds->plan);
     if (ds->d_elems) (void)hipFree(ds->d_elems);
     if (ds->d_sizes) (void)hipFree(ds->d_sizes);
     if (ds->d_offsets) (void)hipFree(ds->d_offsets);

@@ -1,0 +1,136 @@
+"""GPU parity of the batched dispatcher, the bitmap-GEMM path and the
+edge-wise counts against the CPU oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+import oracle
+import set_intersection_amd as si
+from set_intersection_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def tuning(ds, **kw):
+    t = ds.default_tuning()
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batched_zipf_default(ctx, graph):
+    coll = datasets.zipf_collection(600, 1.2, 16, 65536, 1 << 20, 21)
+    ds = ctx.upload(coll)
+    exp, _ = oracle.all_pairs(coll)
+    t = tuning(ds, use_graph=graph)
+    assert np.array_equal(ds.batched_count(t), exp)
+    assert np.array_equal(ds.batched_count(t), exp)  # replay
+
+
+@pytest.mark.parametrize("bitmap_min,chunk,chunk_bm", [(1, 64, 64), (64, 1000, 5000), (100000, 1 << 16, 1 << 20)])
+def test_batched_bins_and_chunks(ctx, bitmap_min, chunk, chunk_bm):
+    """Force every row into the bitmap bin (bitmap_min=1) or the hash bin,
+    with tiny chunks so sets are cut by wave boundaries."""
+    coll = datasets.zipf_collection(300, 1.1, 1, 9000, 1 << 18, 5)
+    ds = ctx.upload(coll)
+    exp, _ = oracle.all_pairs(coll)
+    t = tuning(ds, bitmap_min=bitmap_min, chunk_elems=chunk, chunk_elems_bitmap=chunk_bm, use_graph=0)
+    assert np.array_equal(ds.batched_count(t), exp)
+
+
+@pytest.mark.parametrize("ratio", [1, 4, 1 << 30])
+def test_batched_big_rows_search_and_merge(ctx, ratio):
+    """Universe 2^32 (no LDS bitmap) with rows larger than an LDS hash table:
+    the SEARCH and MERGE bins."""
+    sizes = [60000, 30000, 20000, 9000, 5000, 4097, 300, 20, 3, 1, 0, 0]
+    coll = datasets.sized_collection(sizes, 1 << 20, 8)
+    # spread the values over the full uint32 range, keep them sorted and unique
+    coll = si.Collection(coll.sizes, coll.elements.astype(np.uint64) * 4093 + 7, universe=2**32 - 1)
+    ds = ctx.upload(coll)
+    exp, _ = oracle.all_pairs(coll)
+    t = tuning(ds, ratio_search=ratio, use_graph=0, chunk_elems=4096)
+    assert np.array_equal(ds.batched_count(t), exp)
+
+
+def test_batched_empty_and_tiny(ctx):
+    coll = si.Collection.from_sets([[], [1, 2, 3], [], [2], [3, 4, 2**32 - 1], [2**32 - 1], []])
+    ds = ctx.upload(coll)
+    exp, _ = oracle.all_pairs(coll)
+    assert np.array_equal(ds.batched_count(), exp)
+    one = ctx.upload(si.Collection.from_sets([[1, 2]]))
+    assert one.batched_count().size == 0
+
+
+def test_batched_autotune(ctx):
+    coll = datasets.zipf_collection(500, 1.2, 16, 65536, 1 << 20, 2)
+    ds = ctx.upload(coll)
+    t = ds.autotune(300.0)
+    assert t.version == si._abi.SI_TUNING_VERSION
+    exp, _ = oracle.all_pairs(coll)
+    assert np.array_equal(ds.batched_count(t), exp)
+
+
+# ------------------------------------------------------------ bitmap GEMM --
+@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32, si.MM_POPCOUNT])
+def test_mm_tiles(ctx, variant):
+    coll = datasets.density_collection(300, 1 << 12, 0.01, 0.3, 11)
+    ds = ctx.upload(coll)
+    # self tile (mirrored upper tiles), off-diagonal tile pair, ragged tiles
+    for (a0, a1), (b0, b1) in [((0, 300), (0, 300)), ((0, 130), (130, 300)), ((7, 140), (200, 261)),
+                               ((250, 300), (0, 3))]:
+        A, B = si.tile(0, a0, a1), si.tile(1, b0, b1)
+        got = ds.mm_counts(A, B, variant)
+        exp = oracle.tile_matrix(coll, a0, a1, b0, b1)
+        assert np.array_equal(got, exp), ((a0, a1), (b0, b1))
+
+
+def test_mm_ld_padding(ctx):
+    coll = datasets.density_collection(70, 1 << 10, 0.05, 0.5, 3)
+    ds = ctx.upload(coll)
+    A, B = si.tile(0, 0, 40), si.tile(1, 40, 70)
+    got = ds.mm_counts(A, B, si.MM_I8, ld=50)
+    exp = oracle.tile_matrix(coll, 0, 40, 40, 70)
+    assert np.array_equal(got[:, :30], exp)
+
+
+def test_mm_c4_shape_slice(ctx):
+    """A 1024-set slice of the 8192 x 2^16 density workload (full K)."""
+    coll = datasets.density_collection(1024, 1 << 16, 0.01, 0.10, 20261203)
+    ds = ctx.upload(coll)
+    A = si.tile(0, 0, 1024)
+    got = ds.mm_counts(A, A, si.MM_I8)
+    exp = oracle.tile_matrix(coll, 0, 1024, 0, 1024)
+    assert np.array_equal(got, exp)
+
+
+# ------------------------------------------------------------------ edges --
+def test_edges_rmat_small(ctx):
+    coll = datasets.rmat_collection(12, 16, seed=4)
+    u, v = datasets.edge_list(coll)
+    ds = ctx.upload(coll)
+    got, total = ds.edge_count(u, v)
+    exp, etot = oracle.edges(coll, u, v)
+    assert np.array_equal(got, exp)
+    assert total == etot and total % 3 == 0
+
+
+@pytest.mark.parametrize("bitmap_min,ratio", [(1, 32), (1 << 30, 32), (1 << 30, 1)])
+def test_edges_bins(ctx, bitmap_min, ratio):
+    coll = datasets.rmat_collection(11, 24, seed=9)
+    u, v = datasets.edge_list(coll)
+    ds = ctx.upload(coll)
+    t = tuning(ds, bitmap_min=bitmap_min, ratio_search=ratio, chunk_elems=512)
+    got, total = ds.edge_count(u, v, t)
+    exp, etot = oracle.edges(coll, u, v)
+    assert np.array_equal(got, exp) and total == etot
+
+
+def test_edges_arbitrary_pairs(ctx):
+    coll = datasets.zipf_collection(200, 1.1, 1, 70000, 2**32, 6)  # big rows, no bitmap
+    rng = np.random.default_rng(0)
+    u = rng.integers(0, 200, 3000).astype(np.uint32)
+    v = rng.integers(0, 200, 3000).astype(np.uint32)
+    ds = ctx.upload(coll)
+    got, total = ds.edge_count(u, v)
+    exp, etot = oracle.edges(coll, u, v)
+    assert np.array_equal(got, exp) and total == etot

@@ -12,6 +12,7 @@ import pytest
 import oracle
 import set_intersection_amd as si
 from set_intersection_amd import datasets
+from helpers import run_layout_counts
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -128,26 +129,6 @@ def test_errors(ctx):
 
 
 # ------------------------------------------------------------------ MISI --
-def run_layout_counts(ds, coll, technique, partition, p=None):
-    """All pair counts via the reference's multi-run driver layout."""
-    k = coll.k
-    tiles = [(t, t * partition, min((t + 1) * partition, k)) for t in range((k + partition - 1) // partition)]
-    single = len(tiles) == 1
-    n = k if single else partition
-    res = {}
-    for x in range(len(tiles)):
-        for y in range(x, len(tiles)):
-            A, B = si.tile(*tiles[x]), si.tile(*tiles[y])
-            c = ds.run_count(technique, A, B, n, p)
-            self_join = A.id == B.id
-            for i in range(A.start, A.end):
-                for j in range(i + 1 if self_join else B.start, B.end):
-                    li, lj = i - A.id * n, j - B.id * n
-                    idx = oracle.triangular_index(n, li, lj) if self_join else li * n + lj
-                    res[(i, j)] = int(c[idx])
-    return res
-
-
 @pytest.mark.parametrize("name", ["closed_form", "random_k12", "random_k23"])
 @pytest.mark.parametrize("partition", [3, 5, 10000])
 def test_golden_runs(ctx, name, partition):

@@ -1,0 +1,111 @@
+"""GPU parity of the obs, hi and bitmap kernels against the CPU oracle.
+
+Every count must equal the oracle's bit for bit: SISI pairs (edge cases,
+closed forms, skewed size ratios), MISI runs through the reference's
+multi-run layout for block- and wave-level variants, and bitmap_naive's
+all-pairs counts.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import set_intersection_amd as si
+from set_intersection_amd import datasets
+from helpers import check_all_pairs, edge_case_pairs, run_layout_counts
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SISI_TECHS = [si.TECH_IP, si.TECH_OBS, si.TECH_HI, si.TECH_BITMAP_DYNAMIC]
+RUN_TECHS = [si.TECH_IP, si.TECH_OBS, si.TECH_HI, si.TECH_BITMAP_DYNAMIC]
+
+
+def check_pair(ctx, a, b, techs=SISI_TECHS):
+    coll = si.Collection.from_sets([a, b])
+    ds = ctx.upload(coll)
+    exp = oracle.count(coll.set(0), coll.set(1))
+    for t in techs:
+        assert ds.pair_count(0, 1, t) == exp, t
+        assert ds.pair_count(1, 0, t) == exp, t
+    ds.free()
+
+
+def test_sisi_edge_cases(ctx):
+    for a, b in edge_case_pairs():
+        check_pair(ctx, a, b)
+
+
+@pytest.mark.parametrize("na,nb,U", [(1 << 16, 1 << 16, 1 << 18), (1 << 20, 1 << 10, 1 << 24),
+                                     (3, 1 << 20, 1 << 22), (8191, 8193, 20000), (100_000, 99_999, 2**32)])
+def test_sisi_sizes(ctx, na, nb, U):
+    a = datasets.uniform_sorted(na, U, na + 1)
+    b = datasets.uniform_sorted(nb, U, nb + 2)
+    if U == 2**32:
+        b = np.union1d(b, a[::97]).astype(np.uint32)
+    check_pair(ctx, a, b)
+
+
+def test_sisi_skewed_2p24x2p12(ctx):
+    """A scaled-down copy of the |A|=2^27, |B|=2^16 size ratio."""
+    coll = datasets.sisi_pair(1 << 24, 1 << 13, 2**32, 17)
+    a, b = coll.set(0), coll.set(1)
+    b = np.union1d(b, a[::4096]).astype(np.uint32)
+    check_pair(ctx, a, b)
+
+
+@pytest.mark.parametrize("tech", RUN_TECHS)
+@pytest.mark.parametrize("warp", [False, True])
+@pytest.mark.parametrize("name,partition", [("closed_form", 10000), ("closed_form", 3), ("random_k12", 5),
+                                            ("random_k23", 10000), ("random_k23", 4)])
+def test_golden_runs(ctx, tech, warp, name, partition):
+    if warp and tech not in (si.TECH_OBS, si.TECH_HI):
+        pytest.skip("--warp only selects a kernel for obs and hi")
+    coll = datasets.read_bin(os.path.join(GOLDEN, name + ".bin"))
+    ds = ctx.upload(coll)
+    got = run_layout_counts(ds, coll, tech, partition, si.params(warp=warp))
+    check_all_pairs(got, coll)
+
+
+@pytest.mark.parametrize("tech", RUN_TECHS)
+def test_runs_zipf_collection(ctx, tech):
+    coll = datasets.zipf_collection(400, 1.2, 4, 20000, 1 << 20, 9)
+    ds = ctx.upload(coll)
+    got = run_layout_counts(ds, coll, tech, 150)
+    check_all_pairs(got, coll)
+
+
+@pytest.mark.parametrize("tech", [si.TECH_OBS, si.TECH_HI])
+def test_runs_warp_zipf(ctx, tech):
+    coll = datasets.zipf_collection(200, 1.1, 1, 70000, 1 << 20, 5)  # rows beyond the LDS table size
+    ds = ctx.upload(coll)
+    got = run_layout_counts(ds, coll, tech, 10000, si.params(warp=True))
+    check_all_pairs(got, coll)
+
+
+def test_bitmap_run_global_universe(ctx):
+    """Universe beyond the LDS bitmap (2^22 bits) takes the global-memory path."""
+    coll = datasets.zipf_collection(120, 1.2, 8, 3000, 1 << 22, 4)
+    ds = ctx.upload(coll)
+    got = run_layout_counts(ds, coll, si.TECH_BITMAP_DYNAMIC, 10000)
+    check_all_pairs(got, coll)
+
+
+@pytest.mark.parametrize("name", ["closed_form_small", "random_k12", "random_k23"])
+def test_bitmap_naive_all_pairs(ctx, name):
+    if name == "closed_form_small":
+        coll = si.Collection.from_sets([np.arange(0, 1000, p, dtype=np.uint32) for p in (2, 3, 5, 7, 11)] + [[]])
+    else:
+        coll = datasets.read_bin(os.path.join(GOLDEN, name + ".bin"))
+    ds = ctx.upload(coll)
+    got = ds.all_pairs_bitmap()
+    counts, _ = oracle.all_pairs(coll)
+    assert np.array_equal(got, counts)
+
+
+def test_bitmap_naive_multi_tile(ctx):
+    coll = datasets.density_collection(300, 1 << 14, 0.01, 0.2, 3)  # 5 x 5 tiles of 64 pairs
+    ds = ctx.upload(coll)
+    got = ds.all_pairs_bitmap()
+    counts, _ = oracle.all_pairs(coll)
+    assert np.array_equal(got, counts)
