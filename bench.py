@@ -1,15 +1,22 @@
-"""Benchmark of the SISI intersect-path (merge path) kernels on one MI355X.
+"""Benchmarks of the MI355X set-intersection kernels (one GPU).
 
-Workload (the configuration the headline metric is quoted on): one pair of
-sorted unique uint32 sets, |A| = |B| = 2^28, drawn uniformly from [0, 2^32),
-count plus materialised result list.  A step is one full intersection
-(merge-path kernel + placement kernel) of the HBM-resident pair; the timed
-region covers exactly K steps between two stream events.
+Default workload (the configuration the headline metric is quoted on): SISI
+intersect-path on one pair of sorted unique uint32 sets, |A| = |B| = 2^28,
+drawn uniformly from [0, 2^32), count plus materialised result list.  A step
+is one full intersection (merge-path kernel + placement kernel) of the
+HBM-resident pair; the timed region covers exactly K steps between two stream
+events.  value = algorithmic bytes per step / seconds per step, with
+bytes = 4(|A|+|B|) + 4|A∩B| + 4.
 
-value = algorithmic bytes per step / seconds per step, with
-bytes = 4(|A|+|B|) + 4|A∩B| + 4 (inputs read once, list written once, count).
-
-Prints one JSON line on stdout; progress goes to stderr.
+--workload selects one of the other configurations instead (each prints one
+JSON line of its own):
+  sisi_small  |A| = |B| = 2^20 from [0, 2^24)
+  sisi_skew   |A| = 2^27, |B| = 2^16 from [0, 2^32): ip / obs / hi / bitmap
+  misi        16384 sets, Zipf(1.2) sizes in [16, 65536], universe 2^20:
+              batched dispatcher (default and autotuned) vs plain per-run kernels
+  mm          8192 sets over 2^16, density U[1%,10%]: int8 MFMA vs FP32 vs popcount
+  edges       R-MAT scale 20, edge factor 16: per-edge counts + total
+Progress goes to stderr; stdout carries the JSON line only.
 """
 import argparse
 import json
@@ -24,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+INT8_PEAK_TOPS = 5000.0
 METRIC = "SISI input GB/s vs HBM peak; MISI pairs/sec batched vs plain; exact counts"
 
 
@@ -31,17 +39,47 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(path):
+def read_traffic(name):
     """HBM bytes per launch recorded from rocprofv3 --pmc runs (or None)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(coll, nbytes, min_seconds):
-    """std::set_intersection (the oracle's restatement of
-    std_set_intersection.cpp:68-73) on the same pair, single thread."""
+def timed(ctx, fn, steps, warmup):
+    """Runs fn warmup times, then times exactly `steps` calls between two
+    stream events.  Returns (ms per step, wall seconds)."""
+    for _ in range(warmup):
+        fn()
+    ctx.sync()
+    ctx.kernel_stats_reset()
+    e0 = ctx.record()
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    e1 = ctx.record()
+    ctx.sync()
+    wall = time.perf_counter() - w0
+    return ctx.elapsed_ms(e0, e1) / steps, wall
+
+
+def kavg(ctx, name):
+    ms, n = ctx.kernel_stats(name)
+    return ms / n if n else None
+
+
+def base_record(args, value, unit, ms, config, dtype="u32", higher=True):
+    return {"metric": METRIC, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": higher, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+# ------------------------------------------------------------------- SISI --
+def cpu_sisi(coll, nbytes, min_seconds, label):
+    """std::set_intersection (oracle restatement of std_set_intersection.cpp:68-73),
+    materialising the list, single thread."""
     import oracle
     a, b = coll.set(0), coll.set(1)
     reps, t0 = 0, time.perf_counter()
@@ -52,8 +90,224 @@ def cpu_baseline(coll, nbytes, min_seconds):
         if el >= min_seconds or reps >= 50:
             break
     return {"value": round(nbytes * reps / el / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"full 2^28 x 2^28 pair, std::set_intersection materialising the list, {reps} reps, "
-                      f"{el:.1f} s"}
+            "sample": f"{label}, std::set_intersection materialising the list, {reps} reps, {el:.1f} s"}
+
+
+def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
+    import oracle
+    na, nb, universe = 1 << log2a, 1 << log2b, 1 << log2u
+    log(f"[bench] generating |A|=2^{log2a} |B|=2^{log2b} uniform in [0,2^{log2u}) ...")
+    t0 = time.perf_counter()
+    coll = datasets.sisi_pair(na, nb, universe, args.seed)
+    log(f"[bench] generated in {time.perf_counter() - t0:.1f} s")
+    ctx = si.Context(0)
+    info = ctx.info()
+    log(f"[bench] device {info['name']} CUs={info['multiprocessor_count']}")
+    ds = ctx.upload(coll, validate=True)
+    out = ctx.buffer(min(na, nb) * 4)
+    cnt = ctx.buffer(8)
+    cnt_only = ctx.buffer(8 * 8)
+
+    exp = oracle.materialize(coll.set(0), coll.set(1))
+    ds.pair_materialize_async(0, 1, out, cnt)
+    got_n = int(cnt.download(np.uint64)[0])
+    exact = bool(got_n == exp.size and np.array_equal(out.download(np.uint32, count=got_n), exp))
+    for i, t in enumerate([si.TECH_IP, si.TECH_OBS, si.TECH_HI, si.TECH_BITMAP_DYNAMIC]):
+        ds.pair_count_async(0, 1, cnt_only, offset=8 * i, technique=t)
+    counts = cnt_only.download(np.uint64)[:4].tolist()
+    exact = exact and all(c == exp.size for c in counts)
+    log(f"[bench] |A∩B| = {got_n} (oracle {exp.size}); ip/obs/hi/bitmap counts {counts}; exact={exact}")
+
+    nbytes = 4 * (na + nb) + 4 * exp.size + 4
+    ctx.kernel_timing(True)
+    ms, wall = timed(ctx, lambda: ds.pair_materialize_async(0, 1, out, cnt), args.steps, args.warmup)
+    k_ms = kavg(ctx, "ip_sisi_materialize")
+    c_ms = kavg(ctx, "ip_sisi_compact")
+    value = nbytes / (ms / 1e3) / 1e9
+    kernel_bytes = 4 * (na + nb) + 4 * exp.size
+    achieved = kernel_bytes / (k_ms / 1e3) / 1e9
+
+    techs = {}
+    for name, t, kname in [("ip", si.TECH_IP, "ip_sisi_count"), ("obs", si.TECH_OBS, "obs_sisi_count"),
+                           ("hi", si.TECH_HI, "hi_sisi_count"), ("bitmap_dynamic", si.TECH_BITMAP_DYNAMIC, None)]:
+        cms, _ = timed(ctx, lambda t=t: ds.pair_count_async(0, 1, cnt_only, technique=t), args.steps, args.warmup)
+        techs[name] = {"ms": round(cms, 4), "input_GBps": round(4 * (na + nb) / (cms / 1e3) / 1e9, 2)}
+        if kname:
+            techs[name]["kernel_ms"] = round(kavg(ctx, kname) or 0, 4)
+    log(f"[bench] materialise {ms:.4f} ms/step ({value:.0f} GB/s); merge kernel {k_ms:.4f} ms "
+        f"({achieved:.0f} GB/s), compact {c_ms:.4f} ms; counts: {techs}")
+    cpu = None if args.no_cpu else cpu_sisi(coll, nbytes, args.cpu_seconds, label)
+    rec = base_record(args, value, "GB/s", ms,
+                      {"workload": f"SISI intersect-path count+materialise, |A|=2^{log2a} |B|=2^{log2b} sorted "
+                                   f"unique uint32 uniform in [0,2^{log2u}), seed {args.seed}",
+                       "intersection": int(exp.size), "bytes_per_step": int(nbytes)})
+    rec["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                       "traffic": read_traffic(f"ip_sisi_materialize_2^{log2a}_pmc.md"),
+                       "kernel": "k_sisi_merge<256,16,materialise>", "kernel_ms": round(k_ms, 4),
+                       "compact_kernel_ms": round(c_ms, 4)}
+    rec["cpu_baseline"] = cpu
+    rec["exact"] = exact
+    rec["count_only"] = techs
+    rec["wall_s_timed"] = round(wall, 4)
+    ctx.close()
+    return rec
+
+
+# ------------------------------------------------------------------- MISI --
+def bench_misi(args, si, datasets):
+    import oracle
+    coll = datasets.make_workload("misi_zipf1.2_16384", args.seed)
+    k = coll.k
+    pairs = k * (k - 1) // 2
+    log(f"[bench] MISI k={k} total={coll.elements.size} pairs={pairs}")
+    ctx = si.Context(0)
+    ds = ctx.upload(coll)
+    out = ctx.buffer(pairs * 4)
+    t_def = ds.default_tuning()
+    ds.batched_count_async(out, t_def)  # plan + items
+    ctx.sync()
+    ms_def, _ = timed(ctx, lambda: ds.batched_count_async(out, t_def), args.steps, args.warmup)
+    res = out.download(np.uint32)
+    log(f"[bench] batched default {ms_def:.3f} ms")
+    t0 = time.perf_counter()
+    t_tuned = ds.autotune(args.tune_budget_ms)
+    tune_s = time.perf_counter() - t0
+    ds.batched_count_async(out, t_tuned)
+    ctx.sync()
+    ms_tuned, _ = timed(ctx, lambda: ds.batched_count_async(out, t_tuned), args.steps, args.warmup)
+    res2 = out.download(np.uint32)
+    log(f"[bench] batched autotuned {ms_tuned:.3f} ms ({t_tuned.as_dict()}), tuning took {tune_s:.1f} s")
+    # exactness: full agreement of the two batched runs and a sampled oracle check
+    rng = np.random.default_rng(args.seed)
+    a = rng.integers(0, k, 200_000).astype(np.uint32)
+    b = rng.integers(0, k, 200_000).astype(np.uint32)
+    keep = a != b
+    a, b = np.minimum(a[keep], b[keep]), np.maximum(a[keep], b[keep])
+    exp, _ = oracle.pairs(coll, a, b)
+    idx = (k * (k - 1) // 2) - (k - a.astype(np.int64)) * (k - a.astype(np.int64) - 1) // 2 + b - a - 1
+    exact = bool(np.array_equal(res, res2) and np.array_equal(res[idx], exp))
+    # plain per-technique kernels: the reference driver's runs with partition 10000
+    plain = {}
+    partition = 10000
+    tiles = [(t, t * partition, min((t + 1) * partition, k)) for t in range((k + partition - 1) // partition)]
+    runbuf = ctx.buffer(partition * partition * 4)
+    for name, tech, warp in [("bitmap_dynamic", si.TECH_BITMAP_DYNAMIC, False), ("hi_block", si.TECH_HI, False),
+                             ("obs_block", si.TECH_OBS, False), ("ip", si.TECH_IP, False)]:
+        if args.skip_plain and name != "bitmap_dynamic":
+            continue
+        p = si.params(warp=warp)
+        ctx.sync()
+        e0 = ctx.record()
+        for x in range(len(tiles)):
+            for y in range(x, len(tiles)):
+                ds.run_count_async(tech, si.tile(*tiles[x]), si.tile(*tiles[y]), partition, runbuf, p)
+        e1 = ctx.record()
+        ctx.sync()
+        pms = ctx.elapsed_ms(e0, e1)
+        plain[name] = {"ms": round(pms, 2), "pairs_per_s": round(pairs / (pms / 1e3), 1)}
+        log(f"[bench] plain {name}: {pms:.1f} ms")
+        if pms > args.plain_limit_ms:
+            log("[bench] skipping slower plain kernels")
+            break
+    best_plain = max(v["pairs_per_s"] for v in plain.values())
+    best_ms = min(ms_def, ms_tuned)
+    value = pairs / (best_ms / 1e3)
+    # CPU baseline: oracle std::set_intersection on a bounded sample of rows
+    cpu = None
+    if not args.no_cpu:
+        nth = os.cpu_count() or 1
+        sa = np.repeat(np.arange(0, 64, dtype=np.uint32), k)
+        sb = np.tile(np.arange(k, dtype=np.uint32), 64)
+        keep = sa < sb
+        _, secs = oracle.pairs(coll, sa[keep], sb[keep], "std", threads=nth)
+        cpu = {"value": round(int(keep.sum()) / secs, 1), "unit": "pairs/s", "cores": nth, "kind": "port",
+               "sample": f"rows 0-63 of the collection vs all partners ({int(keep.sum())} pairs), "
+                         f"std::set_intersection, {secs:.1f} s"}
+    rec = base_record(args, value, "pairs/s", best_ms,
+                      {"workload": "MISI all pairs, 16384 sets, Zipf(1.2) sizes clamped [16,65536], elements "
+                                   f"uniform in [0,2^20), seed {args.seed}", "pairs": pairs})
+    rec["batched"] = {"default_ms": round(ms_def, 3), "autotuned_ms": round(ms_tuned, 3),
+                      "tuning": t_tuned.as_dict(), "tune_seconds": round(tune_s, 1)}
+    rec["plain"] = plain
+    rec["speedup_vs_best_plain"] = round(value / best_plain, 2)
+    rec["cpu_baseline"] = cpu
+    rec["exact"] = exact
+    ctx.close()
+    return rec
+
+
+# --------------------------------------------------------------------- mm --
+def bench_mm(args, si, datasets):
+    import oracle
+    coll = datasets.make_workload("mm_8192x2^16_density1-10", args.seed)
+    k = coll.k
+    ctx = si.Context(0)
+    ds = ctx.upload(coll)
+    A = si.tile(0, 0, k)
+    out = ctx.buffer(k * k * 4)
+    res = {}
+    mats = {}
+    ctx.kernel_timing(True)
+    for name, var, kname in [("i8", si.MM_I8, "mm_gemm_i8"), ("f32", si.MM_F32, "mm_gemm_f32"),
+                             ("popcount", si.MM_POPCOUNT, "popcount_pairs")]:
+        ms, _ = timed(ctx, lambda var=var: ds.mm_counts_async(A, A, out, var), args.steps, args.warmup)
+        kms = kavg(ctx, kname)
+        res[name] = {"ms": round(ms, 3), "kernel_ms": round(kms or 0, 3)}
+        mats[name] = out.download(np.uint32)
+        log(f"[bench] mm {name}: {ms:.3f} ms/step, kernel {kms}")
+    # exactness: variants agree everywhere; oracle on a row sample
+    exact = bool(np.array_equal(mats["i8"], mats["f32"]) and np.array_equal(mats["i8"], mats["popcount"]))
+    M = mats["i8"].reshape(k, k)
+    sample = oracle.tile_matrix(coll, 0, 64, 0, k)
+    exact = exact and bool(np.array_equal(M[:64], sample))
+    U = coll.universe + 1
+    Kp = (U + 63) // 64 * 64
+    ops = 2.0 * k * k * Kp / 2  # self tile: upper block triangle computed (+ diagonal blocks)
+    tiles = (k + 127) // 128
+    ops_exec = 2.0 * (tiles * (tiles + 1) // 2) * 128 * 128 * Kp
+    i8k = res["i8"]["kernel_ms"] / 1e3
+    tops = ops_exec / i8k / 1e12
+    rec = base_record(args, 2.0 * k * k * Kp / (res["i8"]["ms"] / 1e3) / 1e12, "TOP/s (dense-equivalent)",
+                      res["i8"]["ms"], {"workload": f"bitmap GEMM, {k} sets over 2^16, density U[1%,10%], "
+                                                    f"seed {args.seed}", "K": Kp}, dtype="i8")
+    rec["variants"] = res
+    rec["roofline"] = {"bound": "mfma", "achieved": round(tops, 2), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                       "frac": round(tops / INT8_PEAK_TOPS, 4), "traffic": None, "kernel": "k_mm_gemm<i8>",
+                       "ops_executed": ops_exec}
+    rec["exact"] = exact
+    rec["cpu_baseline"] = None
+    ctx.close()
+    return rec
+
+
+# ------------------------------------------------------------------ edges --
+def bench_edges(args, si, datasets):
+    import oracle
+    coll = datasets.make_workload("rmat20_ef16", args.seed)
+    u, v = datasets.edge_list(coll)
+    m = u.size
+    log(f"[bench] R-MAT 20: {coll.k} vertices, {coll.elements.size} adjacency entries, {m} edges")
+    ctx = si.Context(0)
+    ds = ctx.upload(coll)
+    t0 = time.perf_counter()
+    plan = ds.edge_plan(u, v)
+    plan_s = time.perf_counter() - t0
+    counts = ctx.buffer(m * 4)
+    total = ctx.buffer(8)
+    ms, _ = timed(ctx, lambda: plan.count_async(counts, total), args.steps, args.warmup)
+    got = counts.download(np.uint32)
+    tot = int(total.download(np.uint64)[0])
+    exp, etot = oracle.edges(coll, u, v)
+    exact = bool(np.array_equal(got, exp) and tot == etot)
+    rec = base_record(args, m / (ms / 1e3), "edges/s", ms,
+                      {"workload": f"R-MAT scale 20, edge factor 16 (a=.57,b=c=.19), undirected, dedup, "
+                                   f"seed {args.seed}", "edges": int(m), "total": tot, "triangles": tot // 3})
+    rec["plan_seconds"] = round(plan_s, 2)
+    rec["exact"] = exact
+    rec["cpu_baseline"] = None
+    ctx.close()
+    return rec
 
 
 def main():
@@ -61,123 +315,48 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log2n", type=int, default=28, help="log2 |A| = log2 |B| (default 28)")
+    ap.add_argument("--workload", default="sisi",
+                    choices=["sisi", "sisi_small", "sisi_skew", "misi", "mm", "edges"])
+    ap.add_argument("--log2n", type=int, default=28, help="sisi: log2 |A| = log2 |B|")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--tune-budget-ms", type=float, default=3000.0)
+    ap.add_argument("--plain-limit-ms", type=float, default=20000.0)
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run the plain copy kernel on MIB MiB (traffic calibration under rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the SISI path shards by value "
-              f"ranges across GPUs but this build runs and reports a single MI355X.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): this build runs and reports a "
+              f"single MI355X; the multi-GPU sharding is described, not run.", file=sys.stderr)
         sys.exit(2)
 
     import set_intersection_amd as si
     from set_intersection_amd import datasets
 
-    n = 1 << args.log2n
-    universe = 1 << 32
-    log(f"[bench] generating |A|=|B|=2^{args.log2n} uniform in [0,2^32) ...")
-    t0 = time.perf_counter()
-    coll = datasets.sisi_pair(n, n, universe, args.seed)
-    log(f"[bench] generated in {time.perf_counter() - t0:.1f} s")
-
-    ctx = si.Context(0)
-    info = ctx.info()
-    log(f"[bench] device {info['name']} CUs={info['multiprocessor_count']}")
-    ds = ctx.upload(coll, validate=True)
-    cap = n
-    out = ctx.buffer(cap * 4)
-    cnt = ctx.buffer(8)
-    cnt_only = ctx.buffer(8)
-
-    # correctness of the measured path against the oracle
-    import oracle
-    exp = oracle.materialize(coll.set(0), coll.set(1))
-    ds.pair_materialize_async(0, 1, out, cnt)
-    got_n = int(cnt.download(np.uint64)[0])
-    got = out.download(np.uint32, count=got_n)
-    exact = bool(got_n == exp.size and np.array_equal(got, exp))
-    ds.pair_count_async(0, 1, cnt_only)
-    exact = exact and int(cnt_only.download(np.uint64)[0]) == exp.size
-    log(f"[bench] |A∩B| = {got_n} (oracle {exp.size}) exact={exact}")
-    del got
-
-    nbytes = 4 * (2 * n) + 4 * exp.size + 4
-
-    # --- timed: count + materialise
-    ctx.kernel_timing(True)
-    for _ in range(args.warmup):
-        ds.pair_materialize_async(0, 1, out, cnt)
-    ctx.sync()
-    ctx.kernel_stats_reset()
-    e0 = ctx.record()
-    w0 = time.perf_counter()
-    for _ in range(args.steps):
-        ds.pair_materialize_async(0, 1, out, cnt)
-    e1 = ctx.record()
-    ctx.sync()
-    wall = time.perf_counter() - w0
-    ms = ctx.elapsed_ms(e0, e1)
-    mat_ms, mat_n = ctx.kernel_stats("ip_sisi_materialize")
-    cmp_ms, cmp_n = ctx.kernel_stats("ip_sisi_compact")
-    step_s = ms / 1e3 / args.steps
-    value = nbytes / step_s / 1e9
-    k_avg = mat_ms / max(mat_n, 1) / 1e3
-    kernel_bytes = 4 * (2 * n) + 4 * exp.size  # inputs once + staged list once
-    achieved = kernel_bytes / k_avg / 1e9
-
-    # --- count only (the reference ip executable's output)
-    for _ in range(args.warmup):
-        ds.pair_count_async(0, 1, cnt_only)
-    ctx.sync()
-    ctx.kernel_stats_reset()
-    c0 = ctx.record()
-    for _ in range(args.steps):
-        ds.pair_count_async(0, 1, cnt_only)
-    c1 = ctx.record()
-    ctx.sync()
-    cms = ctx.elapsed_ms(c0, c1) / args.steps
-    ck_ms, ck_n = ctx.kernel_stats("ip_sisi_count")
-    count_gbps = (4 * 2 * n + 8) / (cms / 1e3) / 1e9
-    log(f"[bench] materialise {ms / args.steps:.3f} ms/step ({value:.0f} GB/s), kernel {k_avg * 1e3:.3f} ms "
-        f"({achieved:.0f} GB/s), compact {cmp_ms / max(cmp_n, 1):.3f} ms; count-only {cms:.3f} ms "
-        f"({count_gbps:.0f} GB/s)")
-
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] CPU baseline ...")
-        cpu = cpu_baseline(coll, nbytes, args.cpu_seconds)
-        log(f"[bench] CPU {cpu['value']} GB/s")
-
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "ip_sisi_materialize_2^28_pmc.md"))
-    rec = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic",
-        "config": {"workload": f"SISI intersect-path count+materialise, |A|=|B|=2^{args.log2n} sorted unique "
-                               f"uint32 uniform in [0,2^32), seed {args.seed}",
-                   "intersection": int(exp.size), "bytes_per_step": int(nbytes)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": "k_sisi_merge<256,16,materialise>", "kernel_ms": round(k_avg * 1e3, 4)},
-        "cpu_baseline": cpu,
-        "exact": exact,
-        "count_only": {"ms": round(cms, 4), "GBps": round(count_gbps, 2),
-                       "kernel_ms": round(ck_ms / max(ck_n, 1), 4)},
-        "wall_s_timed": round(wall, 4),
-    }
+    if args.calibrate_copy:
+        ctx = si.Context(0)
+        nb = args.calibrate_copy << 20
+        src, dst = ctx.buffer(nb), ctx.buffer(nb)
+        for _ in range(3):
+            dst.copy_from(src, nb)
+        ctx.sync()
+        ctx.close()
+    if args.workload == "sisi":
+        rec = bench_sisi(args, si, datasets, args.log2n, args.log2n, 32, f"full 2^{args.log2n} x 2^{args.log2n} pair")
+    elif args.workload == "sisi_small":
+        rec = bench_sisi(args, si, datasets, 20, 20, 24, "full 2^20 x 2^20 pair")
+    elif args.workload == "sisi_skew":
+        rec = bench_sisi(args, si, datasets, 27, 16, 32, "full 2^27 x 2^16 pair")
+    elif args.workload == "misi":
+        rec = bench_misi(args, si, datasets)
+    elif args.workload == "mm":
+        rec = bench_mm(args, si, datasets)
+    else:
+        rec = bench_edges(args, si, datasets)
     print(json.dumps(rec), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -134,6 +134,9 @@ int si_buffer_device_ptr(si_buffer* buf, void** dptr);
 int si_buffer_download(si_buffer* buf, uint64_t offset, void* host, uint64_t bytes);
 int si_buffer_upload(si_buffer* buf, uint64_t offset, const void* host, uint64_t bytes);
 int si_buffer_zero(si_buffer* buf);
+/* Device-to-device copy of `bytes` (multiple of 16) by a plain streaming
+ * kernel ("buffer_copy"); its HBM counters calibrate traffic measurements. */
+int si_buffer_copy(si_buffer* dst, const si_buffer* src, uint64_t bytes);
 
 /* ---------------------------------------------------------------- dataset -- */
 typedef struct si_dataset si_dataset;

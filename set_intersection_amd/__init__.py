@@ -6,6 +6,7 @@ binding used by the tests and bench.py.  Every call goes to the HIP kernels;
 a missing library raises LibraryMissing and there is no CPU fallback.
 """
 import ctypes
+import weakref
 from ctypes import byref, c_double, c_float, c_int32, c_uint32, c_uint64, c_void_p
 
 import numpy as np
@@ -60,6 +61,7 @@ class Buffer:
         self.nbytes = int(nbytes)
         self.h = c_void_p()
         _check(_abi.lib().si_buffer_alloc(ctx.h, self.nbytes, byref(self.h)))
+        ctx._children.add(self)
 
     def download(self, dtype=np.uint32, count=None, offset=0):
         dtype = np.dtype(dtype)
@@ -76,10 +78,13 @@ class Buffer:
     def zero(self):
         _check(_abi.lib().si_buffer_zero(self.h))
 
+    def copy_from(self, src, nbytes):
+        _check(_abi.lib().si_buffer_copy(self.h, src.h, nbytes))
+
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             _abi.lib().si_buffer_free(self.h)
-            self.h = c_void_p()
+        self.h = c_void_p()
 
     def __del__(self):
         try:
@@ -99,14 +104,15 @@ class EdgePlan:
         self.h = c_void_p()
         _check(_abi.lib().si_edge_plan_create(ds.h, _u32p(u), _u32p(v), u.size, byref(tuning) if tuning else None,
                                               byref(self.h)))
+        ds.ctx._children.add(self)
 
     def count_async(self, counts, total):
         _check(_abi.lib().si_edge_count_async(self.h, counts.h, total.h))
 
     def free(self):
-        if self.h:
+        if self.h and self.ds.h and self.ds.ctx.h:
             _abi.lib().si_edge_plan_free(self.h)
-            self.h = c_void_p()
+        self.h = c_void_p()
 
     def __del__(self):
         try:
@@ -120,6 +126,7 @@ class Context:
 
     def __init__(self, device=0):
         self.h = c_void_p()
+        self._children = weakref.WeakSet()
         _check(_abi.lib().si_open(device, byref(self.h)))
 
     def info(self):
@@ -161,9 +168,21 @@ class Context:
         return Dataset(self, coll, validate)
 
     def close(self):
+        """Frees every buffer, dataset and plan created from this context, then
+        the context itself."""
         if self.h:
+            for kind in (EdgePlan, Dataset, Buffer):
+                for child in list(self._children):
+                    if isinstance(child, kind):
+                        child.free()
             _abi.lib().si_close(self.h)
             self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
@@ -184,6 +203,7 @@ class Dataset:
         _check(_abi.lib().si_dataset_upload(ctx.h, len(sizes), int(coll.universe), int(elems.size), _u32p(sizes),
                                             _u32p(elems), int(bool(validate)), byref(self.h)))
         self.k = len(sizes)
+        ctx._children.add(self)
 
     # ---- SISI
     def pair_count(self, a, b, technique=TECH_IP, p=None):
@@ -262,7 +282,8 @@ class Dataset:
 
     def free(self):
         if self.h:
-            _abi.lib().si_dataset_free(self.h)
+            if self.ctx.h:
+                _abi.lib().si_dataset_free(self.h)
             self.h = c_void_p()
 
     def __del__(self):

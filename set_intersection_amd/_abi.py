@@ -72,6 +72,7 @@ SIGNATURES = {
     "si_buffer_download": (c_int, [P, c_uint64, c_void_p, c_uint64]),
     "si_buffer_upload": (c_int, [P, c_uint64, c_void_p, c_uint64]),
     "si_buffer_zero": (c_int, [P]),
+    "si_buffer_copy": (c_int, [P, P, c_uint64]),
     "si_dataset_upload": (c_int, [P, c_uint32, c_uint32, c_uint64, U32P, U32P, c_int32, POINTER(P)]),
     "si_dataset_free": (c_int, [P]),
     "si_dataset_info": (c_int, [P, U32P, U32P, U64P]),

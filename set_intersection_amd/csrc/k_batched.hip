@@ -843,6 +843,7 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
     plan->n_bm = uint32_t(bm.size());
     plan->n_pr = uint32_t(pr.size());
     plan->n_merge = uint32_t(merge.size());
+    ds->ctx->edge_plans.insert(plan.get());
     *out = plan.release();
     return SI_OK;
 }
@@ -879,8 +880,14 @@ int edge_count(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
     return SI_OK;
 }
 
-void edge_plan_free(si_edge_plan* p) { delete p; }
+void edge_plan_free(si_edge_plan* p) {
+    p->ds->ctx->edge_plans.erase(p);
+    (void)hipStreamSynchronize(p->ds->ctx->stream);
+    delete p;
+}
 
 uint64_t plan_edges(si_edge_plan* p) { return p->m; }
+
+si_dataset* plan_dataset(si_edge_plan* p) { return p->ds; }
 
 }  // namespace si_impl

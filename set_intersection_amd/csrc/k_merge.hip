@@ -137,17 +137,25 @@ __global__ __launch_bounds__(WAVE) void k_run_merge(const uint32_t* __restrict__
     }
 }
 
-constexpr int SISI_NT = 256;
-constexpr int SISI_VT = 16;
 constexpr int RUN_VT = 8;
 
 }  // namespace
 
 namespace si_impl {
 
-static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, uint32_t& nseg, uint64_t& seg_len) {
-    constexpr uint64_t T = uint64_t(SISI_NT) * SISI_VT;
-    uint64_t maxSeg = uint64_t(ctx->num_cus) * 4;
+// Tile shapes of the SISI merge kernel (threads x elements per thread) and
+// resident workgroups per CU; shape 0 is the default, the others are
+// selectable through si_params.reserved[0] for tuning sweeps.
+struct SisiShape {
+    int nt, vt, per_cu;
+};
+static const SisiShape kShapes[] = {{256, 16, 2}, {256, 8, 4}, {512, 8, 2}, {128, 16, 4}, {256, 16, 4}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, const SisiShape& sh, uint32_t& nseg,
+                          uint64_t& seg_len) {
+    const uint64_t T = uint64_t(sh.nt) * sh.vt;
+    uint64_t maxSeg = uint64_t(ctx->num_cus) * sh.per_cu;
     if (p && p->blocks) maxSeg = p->blocks;
     uint64_t want = (n + T - 1) / T;
     if (want < 1) want = 1;
@@ -158,21 +166,47 @@ static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, uint3
     if (nseg == 0) nseg = 1;
 }
 
+template <int NT, int VT, bool MAT>
+static void launch_sisi(si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA, uint64_t oB,
+                        uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done, uint64_t* out,
+                        uint32_t* out32, uint32_t* stage, uint64_t cap) {
+    hipLaunchKernelGGL((k_sisi_merge<NT, VT, MAT>), dim3(nseg), dim3(NT), 0, ctx->stream, base, oA, nA, oB, nB, seg_len,
+                       nseg, partials, done, out, out32, stage, cap);
+}
+
+template <bool MAT>
+static void launch_shape(int shape, si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA,
+                         uint64_t oB, uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done, uint64_t* out,
+                         uint32_t* out32, uint32_t* stage, uint64_t cap) {
+    switch (shape) {
+        case 1: launch_sisi<256, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        case 2: launch_sisi<512, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        case 3: launch_sisi<128, 16, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        default: launch_sisi<256, 16, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+    }
+}
+
+static int shape_of(const si_params* p) {
+    const int s = p ? int(p->reserved[0]) : 0;
+    return (s >= 0 && s < kNumShapes) ? s : 0;
+}
+
 int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32) {
     si_context* ctx = ds->ctx;
     const uint64_t oA = ds->offsets[a], nA = ds->sizes[a];
     const uint64_t oB = ds->offsets[b], nB = ds->sizes[b];
+    const int shape = shape_of(p);
     uint32_t nseg;
     uint64_t seg_len;
-    sisi_geometry(ctx, nA + nB, p, nseg, seg_len);
+    sisi_geometry(ctx, nA + nB, p, kShapes[shape], nseg, seg_len);
     void* ws = nullptr;
     SI_TRY(small_workspace(ctx, 256 + uint64_t(nseg) * 8, &ws));
     uint32_t* done = static_cast<uint32_t*>(ws);
     uint64_t* partials = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256);
     if (nA == 0 || nB == 0) return zero_outputs(ctx, d_out, d_out32);
     KernelScope ks(ctx, "ip_sisi_count");
-    hipLaunchKernelGGL((k_sisi_merge<SISI_NT, SISI_VT, false>), dim3(nseg), dim3(SISI_NT), 0, ctx->stream,
-                       ds->d_elems, oA, nA, oB, nB, seg_len, nseg, partials, done, d_out, d_out32, nullptr, uint64_t(0));
+    launch_shape<false>(shape, ctx, nseg, ds->d_elems, oA, nA, oB, nB, seg_len, partials, done, d_out, d_out32,
+                        nullptr, 0);
     SI_HIP(hipGetLastError());
     return SI_OK;
 }
@@ -185,9 +219,10 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
         SI_HIP(hipMemsetAsync(d_count, 0, 8, ctx->stream));
         return SI_OK;
     }
+    const int shape = ctx->sisi_shape >= 0 && ctx->sisi_shape < kNumShapes ? ctx->sisi_shape : 0;
     uint32_t nseg;
     uint64_t seg_len;
-    sisi_geometry(ctx, nA + nB, nullptr, nseg, seg_len);
+    sisi_geometry(ctx, nA + nB, nullptr, kShapes[shape], nseg, seg_len);
     const uint64_t cap = ((seg_len / 2 + 2) + 63) & ~uint64_t(63);
     void* ws = nullptr;
     SI_TRY(small_workspace(ctx, 256 + uint64_t(nseg) * 8, &ws));
@@ -197,8 +232,8 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
     uint32_t* stage = ctx->ws_stage.as<uint32_t>();
     {
         KernelScope ks(ctx, "ip_sisi_materialize");
-        hipLaunchKernelGGL((k_sisi_merge<SISI_NT, SISI_VT, true>), dim3(nseg), dim3(SISI_NT), 0, ctx->stream,
-                           ds->d_elems, oA, nA, oB, nB, seg_len, nseg, partials, done, d_count, nullptr, stage, cap);
+        launch_shape<true>(shape, ctx, nseg, ds->d_elems, oA, nA, oB, nB, seg_len, partials, done, d_count, nullptr,
+                           stage, cap);
         SI_HIP(hipGetLastError());
     }
     {

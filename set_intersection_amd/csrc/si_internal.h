@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,11 @@ struct si_context {
     si_impl::DevMem ws_aux;     // technique-specific (bitmaps, hash bins, pair lists)
     si_impl::DevMem ws_aux2;
     bool ws_small_zeroed = false;
+    int sisi_shape = 0;  // tile shape of the SISI merge kernel used by materialisation
+    // objects created from this context; si_close frees whatever is left
+    std::set<si_buffer*> buffers;
+    std::set<si_dataset*> datasets;
+    std::set<si_edge_plan*> edge_plans;
 };
 
 struct si_buffer {
@@ -145,5 +151,6 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
 int edge_count(si_edge_plan* plan, uint32_t* d_counts, uint64_t* d_total);
 void edge_plan_free(si_edge_plan* plan);
 uint64_t plan_edges(si_edge_plan* plan);
+si_dataset* plan_dataset(si_edge_plan* plan);
 
 }  // namespace si_impl

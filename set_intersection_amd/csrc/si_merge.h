@@ -20,14 +20,17 @@ template <int NT, int VT>
 struct MergeCfg {
     static constexpr int T = NT * VT;          // merged elements per tile
     static constexpr uint32_t M = T - 1;       // circular window mask
+    static constexpr int B0 = 2 * T;           // start of the B window in MergeSmem::w
     static constexpr int CPT = (T / 4 + 4 + NT - 1) / NT;  // 16-byte chunks per thread per refill
     static_assert((T & (T - 1)) == 0, "tile must be a power of two");
 };
 
 template <int NT, int VT>
 struct MergeSmem {
-    uint32_t wa[NT * VT];
-    uint32_t wb[NT * VT];
+    // Circular windows of T elements each, stored twice (slot s and s + T) so
+    // that T consecutive elements from any start slot are contiguous: A in
+    // w[0, 2T), B in w[2T, 4T).
+    uint32_t w[4 * NT * VT];
     uint32_t ends[NT];
     uint32_t wave_tot[NT / WAVE];
     uint64_t red[NT / WAVE];
@@ -35,43 +38,63 @@ struct MergeSmem {
 };
 
 // Loads of one window refill, held in registers across the tile's merge.
+// Element ranges are absolute indices into the element array; per-lane work
+// uses 32-bit offsets from uniform (scalar) base pointers.
 template <int NT, int VT>
 struct Refill {
     using C = MergeCfg<NT, VT>;
     uint4 v[C::CPT];
-    uint64_t cA0, cB0;     // first chunk index of the A / B ranges
-    uint32_t nCA, nCB;     // chunk counts
-    uint64_t gA0, gA1, gB0, gB1;  // element ranges [g0, g1) to write
+    const uint4* pA;
+    const uint4* pB;
+    uint32_t nCA, nCB;        // chunk counts
+    uint32_t phA, phB;        // offset of the first needed element in its chunk
+    uint32_t lenA, lenB;      // needed elements
+    uint32_t sA, sB;          // LDS slot of the first chunk's first element
 
     __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t a0, uint64_t a1, uint64_t b0,
                                           uint64_t b1) {
-        gA0 = a0;
-        gA1 = a1;
-        gB0 = b0;
-        gB1 = b1;
-        cA0 = a0 >> 2;
-        nCA = a1 > a0 ? uint32_t(((a1 + 3) >> 2) - cA0) : 0u;
-        cB0 = b0 >> 2;
-        nCB = b1 > b0 ? uint32_t(((b1 + 3) >> 2) - cB0) : 0u;
-        const uint4* __restrict__ b4 = reinterpret_cast<const uint4*>(base);
+        const uint4* b4 = reinterpret_cast<const uint4*>(base);
+        lenA = a1 > a0 ? uint32_t(a1 - a0) : 0u;
+        lenB = b1 > b0 ? uint32_t(b1 - b0) : 0u;
+        nCA = lenA ? uint32_t(((a1 + 3) >> 2) - (a0 >> 2)) : 0u;
+        nCB = lenB ? uint32_t(((b1 + 3) >> 2) - (b0 >> 2)) : 0u;
+        phA = uint32_t(a0 & 3);
+        phB = uint32_t(b0 & 3);
+        sA = uint32_t(a0 & ~3ull) & C::M;
+        sB = uint32_t(b0 & ~3ull) & C::M;
+        pA = b4 + (a0 >> 2);
+        pB = b4 + (b0 >> 2);
+        // every lane issues all CPT loads back to back: out-of-range chunk
+        // numbers are clamped to the last needed chunk (in bounds, not committed)
+        const uint32_t tot = nCA + nCB;
+        if (tot) {
 #pragma unroll
-        for (int j = 0; j < C::CPT; ++j) {
-            const uint32_t c = threadIdx.x + uint32_t(j) * NT;
-            if (c < nCA)
-                v[j] = b4[cA0 + c];
-            else if (c < nCA + nCB)
-                v[j] = b4[cB0 + (c - nCA)];
+            for (int j = 0; j < C::CPT; ++j) {
+                uint32_t c = threadIdx.x + uint32_t(j) * NT;
+                c = c < tot ? c : tot - 1;
+                const uint4* p = c < nCA ? pA + c : pB + (c - nCA);
+                v[j] = *p;
+            }
         }
     }
 
-    __device__ __forceinline__ static void put(uint32_t* w, const uint4& x, uint64_t g, uint64_t lo, uint64_t hi) {
-        if (g >= lo && g + 4 <= hi) {
-            *reinterpret_cast<uint4*>(&w[uint32_t(g) & C::M]) = x;
+    // chunk c of a range: elements rel = 4c - ph + q, valid for 0 <= rel < len;
+    // written to slot and to its mirror slot + T
+    __device__ __forceinline__ static void put(uint32_t* w, const uint4& x, uint32_t c, uint32_t ph, uint32_t len,
+                                               uint32_t s0) {
+        const int rel = int(4 * c) - int(ph);
+        const uint32_t slot = (s0 + 4 * c) & C::M;
+        if (rel >= 0 && rel + 4 <= int(len)) {
+            *reinterpret_cast<uint4*>(&w[slot]) = x;
+            *reinterpret_cast<uint4*>(&w[slot + C::T]) = x;
         } else {
-            if (g + 0 >= lo && g + 0 < hi) w[uint32_t(g + 0) & C::M] = x.x;
-            if (g + 1 >= lo && g + 1 < hi) w[uint32_t(g + 1) & C::M] = x.y;
-            if (g + 2 >= lo && g + 2 < hi) w[uint32_t(g + 2) & C::M] = x.z;
-            if (g + 3 >= lo && g + 3 < hi) w[uint32_t(g + 3) & C::M] = x.w;
+            const uint32_t e[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (rel + q >= 0 && rel + q < int(len)) {
+                    w[slot + q] = e[q];
+                    w[slot + q + C::T] = e[q];
+                }
         }
     }
 
@@ -80,9 +103,9 @@ struct Refill {
         for (int j = 0; j < C::CPT; ++j) {
             const uint32_t c = threadIdx.x + uint32_t(j) * NT;
             if (c < nCA)
-                put(sm.wa, v[j], (cA0 + c) * 4, gA0, gA1);
+                put(sm.w, v[j], c, phA, lenA, sA);
             else if (c < nCA + nCB)
-                put(sm.wb, v[j], (cB0 + (c - nCA)) * 4, gB0, gB1);
+                put(sm.w + C::B0, v[j], c - nCA, phB, lenB, sB);
         }
     }
 };
@@ -121,10 +144,13 @@ __device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA
         const uint32_t Tc = uint32_t((dEnd - d) < T ? (dEnd - d) : T);
         const uint32_t nAw = uint32_t((nA - ia) < T ? (nA - ia) : T);
         const uint32_t nBw = uint32_t((nB - ib) < T ? (nB - ib) : T);
-        const uint32_t gA = uint32_t(oA + ia);  // slot base (mod T) of the windows
-        const uint32_t gB = uint32_t(oB + ib);
-        auto WA = [&](uint32_t i) { return sm.wa[(gA + i) & C::M]; };
-        auto WB = [&](uint32_t i) { return sm.wb[(gB + i) & C::M]; };
+        // window bases: A[ia + i] is at wA[i], B[ib + i] at wB[i], for i <= T
+        const uint32_t* wA = sm.w + (uint32_t(oA + ia) & C::M);
+        const uint32_t* wB = sm.w + C::B0 + (uint32_t(oB + ib) & C::M);
+        auto WA = [&](uint32_t i) { return wA[i]; };
+        auto WB = [&](uint32_t i) { return wB[i]; };
+        // interior tile: full windows and a full tile need no bounds checks
+        const bool full = (Tc == T) && (nAw == T) && (nBw == T);
 
         // 1. split at the end of this thread's diagonal range
         const uint32_t dd = ((tid + 1) * VT < Tc) ? (tid + 1) * VT : Tc;
@@ -161,29 +187,54 @@ __device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA
 
         // 3. merge this thread's VT steps; A is taken on ties and matched
         //    against the current B head (which may lie past y1).
-        uint32_t x = x0, y = y0;
-        uint32_t av = x < x1 ? WA(x) : 0u;
-        uint32_t bv = y < nBw ? WB(y) : 0u;
-        const uint32_t steps = (x1 - x0) + (y1 - y0);
         uint32_t cnt = 0;
         uint32_t mval[MAT ? VT : 1];
         uint32_t mbits = 0;
+        if (full) {
+            // Every thread has exactly VT steps and the true heads A[x], B[y]
+            // are in the windows for x <= x1, y <= y1 (x1 == T or y1 == T only
+            // when the other side has no steps), so the merge-path order alone
+            // decides: take A iff A[x] <= B[y]; a tie is a match.
+            const uint32_t* pa = wA + x0;
+            const uint32_t* pb = wB + y0;
+            uint32_t av = *pa, bv = *pb;
 #pragma unroll
-        for (int k = 0; k < VT; ++k) {
-            if (uint32_t(k) < steps) {
-                const bool takeA = (x < x1) && (y >= y1 || av <= bv);
-                const bool hit = takeA && (y < nBw) && (av == bv);
+            for (int k = 0; k < VT; ++k) {
+                const bool takeA = av <= bv;
+                const bool hit = av == bv;
                 cnt += hit;
                 if (MAT) {
                     mval[k] = av;
                     mbits |= uint32_t(hit) << k;
                 }
-                if (takeA) {
-                    ++x;
-                    if (x < x1) av = WA(x);
-                } else {
-                    ++y;
-                    if (y < nBw) bv = WB(y);
+                pa += takeA;
+                pb += !takeA;
+                const uint32_t nv = *(takeA ? pa : pb);
+                av = takeA ? nv : av;
+                bv = takeA ? bv : nv;
+            }
+        } else {
+            uint32_t x = x0, y = y0;
+            uint32_t av = x < x1 ? WA(x) : 0u;
+            uint32_t bv = y < nBw ? WB(y) : 0u;
+            const uint32_t steps = (x1 - x0) + (y1 - y0);
+#pragma unroll
+            for (int k = 0; k < VT; ++k) {
+                if (uint32_t(k) < steps) {
+                    const bool takeA = (x < x1) && (y >= y1 || av <= bv);
+                    const bool hit = takeA && (y < nBw) && (av == bv);
+                    cnt += hit;
+                    if (MAT) {
+                        mval[k] = av;
+                        mbits |= uint32_t(hit) << k;
+                    }
+                    if (takeA) {
+                        ++x;
+                        if (x < x1) av = WA(x);
+                    } else {
+                        ++y;
+                        if (y < nBw) bv = WB(y);
+                    }
                 }
             }
         }
@@ -208,10 +259,10 @@ __device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA
                 if (i < int(w)) before += t;
                 wtot += t;
             }
-            uint64_t pos = written + before + excl;
+            uint32_t* dst = stage + written + before + excl;
 #pragma unroll
             for (int k = 0; k < VT; ++k) {
-                if (mbits & (1u << k)) stage[pos++] = mval[k];
+                if (mbits & (1u << k)) *dst++ = mval[k];
             }
             written += wtot;
         }
@@ -222,6 +273,5 @@ __device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA
     }
     return myCount;
 }
-
 
 }  // namespace si_dev

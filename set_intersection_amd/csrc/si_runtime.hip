@@ -158,6 +158,11 @@ __global__ void k_check_sorted(const uint32_t* __restrict__ elems, const uint64_
     }
 }
 
+__global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -188,6 +193,10 @@ int si_close(si_context* ctx) {
     if (!ctx) return SI_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    // handles created from the context die with it
+    while (!ctx->edge_plans.empty()) si_edge_plan_free(*ctx->edge_plans.begin());
+    while (!ctx->datasets.empty()) si_dataset_free(*ctx->datasets.begin());
+    while (!ctx->buffers.empty()) si_buffer_free(*ctx->buffers.begin());
     drain_pending(ctx);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
     for (auto e : ctx->free_events) (void)hipEventDestroy(e);
@@ -285,12 +294,14 @@ int si_buffer_alloc(si_context* ctx, uint64_t bytes, si_buffer** out) {
         delete b;
         return fail(SI_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
     }
+    ctx->buffers.insert(b);
     *out = b;
     return SI_OK;
 }
 
 int si_buffer_free(si_buffer* b) {
     if (!b) return SI_OK;
+    b->ctx->buffers.erase(b);
     (void)hipStreamSynchronize(b->ctx->stream);
     (void)hipFree(b->ptr);
     delete b;
@@ -324,6 +335,19 @@ int si_buffer_upload(si_buffer* b, uint64_t off, const void* host, uint64_t byte
 int si_buffer_zero(si_buffer* b) {
     if (!b) return fail(SI_ERR_ARG, "null buffer");
     SI_HIP(hipMemsetAsync(b->ptr, 0, b->bytes, b->ctx->stream));
+    return SI_OK;
+}
+
+int si_buffer_copy(si_buffer* dst, const si_buffer* src, uint64_t bytes) {
+    if (!dst || !src) return fail(SI_ERR_ARG, "null buffer");
+    if ((bytes & 15) || bytes > dst->bytes || bytes > src->bytes) return fail(SI_ERR_RANGE, "bad copy size");
+    si_context* ctx = dst->ctx;
+    const uint64_t n16 = bytes / 16;
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((n16 + 255) / 256, uint64_t(ctx->num_cus) * 32)));
+    KernelScope ks(ctx, "buffer_copy");
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, ctx->stream, static_cast<const uint4*>(src->ptr),
+                       static_cast<uint4*>(dst->ptr), n16);
+    SI_HIP(hipGetLastError());
     return SI_OK;
 }
 
@@ -390,12 +414,19 @@ int si_dataset_upload(si_context* ctx, uint32_t k, uint32_t universe, uint64_t t
     }
     e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("dataset upload: ") + hipGetErrorString(e));
+    ctx->datasets.insert(ds);
     *out = ds;
     return SI_OK;
 }
 
 int si_dataset_free(si_dataset* ds) {
     if (!ds) return SI_OK;
+    ds->ctx->datasets.erase(ds);
+    // edge plans read the dataset: they go first
+    std::vector<si_edge_plan*> dependent;
+    for (si_edge_plan* p : ds->ctx->edge_plans)
+        if (si_impl::plan_dataset(p) == ds) dependent.push_back(p);
+    for (si_edge_plan* p : dependent) si_edge_plan_free(p);
     (void)hipStreamSynchronize(ds->ctx->stream);
     if (ds->plan) free_batch_plan(ds->plan);
     if (ds->d_elems) (void)hipFree(ds->d_elems);
