@@ -40,13 +40,13 @@ using namespace si_dev;
 //   partials[s]     per-segment count (MAT) / partial sum (count mode)
 //   done            per-launch ticket counter, left at zero
 //   out             final uint64 count (count mode)
-template <int NT, int VT, bool MAT>
+template <int NT, int VT, bool MAT, bool DUAL = false>
 __global__ __launch_bounds__(NT) void k_sisi_merge(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
                                                    uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
                                                    uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
                                                    uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
                                                    uint32_t* __restrict__ stage, uint64_t stage_cap) {
-    __shared__ MergeSmem<NT, VT> sm;
+    __shared__ RingSmem<NT, VT> sm;
     const uint32_t s = blockIdx.x;
     const uint64_t n = nA + nB;
     const uint64_t dBeg = uint64_t(s) * seg_len;
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_sisi_merge(const uint32_t* __restrict__ 
     const uint64_t ia = sm.seg[0], iaE = sm.seg[1];
     const uint64_t ib = dBeg - ia, ibE = dEnd - iaE;
 
-    uint64_t c = merge_segment<NT, VT, MAT>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm,
+    uint64_t c = merge_segment_pf<NT, VT, MAT, DUAL>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm,
                                             MAT ? stage + uint64_t(s) * stage_cap : nullptr);
     const uint64_t total = block_sum_u64<NT>(c, sm.red);
     if (threadIdx.x == 0) partials[s] = total;
@@ -149,7 +149,8 @@ namespace si_impl {
 struct SisiShape {
     int nt, vt, per_cu;
 };
-static const SisiShape kShapes[] = {{256, 16, 2}, {256, 8, 4}, {512, 8, 2}, {128, 16, 4}, {256, 16, 4}};
+static const SisiShape kShapes[] = {{256, 8, 4}, {256, 8, 4}, {512, 8, 2}, {128, 16, 4},
+                                     {128, 8, 8}, {256, 16, 2}, {128, 8, 8}};  // 1, 6: dual-read merge loop
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, const SisiShape& sh, uint32_t& nseg,
@@ -166,11 +167,11 @@ static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, const
     if (nseg == 0) nseg = 1;
 }
 
-template <int NT, int VT, bool MAT>
+template <int NT, int VT, bool MAT, bool DUAL = false>
 static void launch_sisi(si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA, uint64_t oB,
                         uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done, uint64_t* out,
                         uint32_t* out32, uint32_t* stage, uint64_t cap) {
-    hipLaunchKernelGGL((k_sisi_merge<NT, VT, MAT>), dim3(nseg), dim3(NT), 0, ctx->stream, base, oA, nA, oB, nB, seg_len,
+    hipLaunchKernelGGL((k_sisi_merge<NT, VT, MAT, DUAL>), dim3(nseg), dim3(NT), 0, ctx->stream, base, oA, nA, oB, nB, seg_len,
                        nseg, partials, done, out, out32, stage, cap);
 }
 
@@ -179,10 +180,13 @@ static void launch_shape(int shape, si_context* ctx, uint32_t nseg, const uint32
                          uint64_t oB, uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done, uint64_t* out,
                          uint32_t* out32, uint32_t* stage, uint64_t cap) {
     switch (shape) {
-        case 1: launch_sisi<256, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        case 1: launch_sisi<256, 8, MAT, true>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
         case 2: launch_sisi<512, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
         case 3: launch_sisi<128, 16, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
-        default: launch_sisi<256, 16, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        case 4: launch_sisi<128, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        case 5: launch_sisi<256, 16, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        case 6: launch_sisi<128, 8, MAT, true>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        default: launch_sisi<256, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
     }
 }
 

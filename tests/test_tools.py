@@ -136,3 +136,20 @@ def test_generate_workload_sisi(tmp_path):
     for i in range(2):
         s = c.set(i).astype(np.int64)
         assert np.all(np.diff(s) > 0) and s[-1] < (1 << 24)
+
+
+def test_degree_cdf(tmp_path):
+    src = os.path.join(GOLDEN, "random_k12.bin")
+    r = run([os.path.join(BIN, "degree_cdf"), src, str(tmp_path / "s")])
+    assert r.returncode == 0, r.stdout
+    c = datasets.read_bin(src)
+    deg = np.bincount(c.elements, minlength=c.universe + 1)
+    read = lambda n: np.loadtxt(str(tmp_path / f"s_{n}.txt"), dtype=np.int64, ndmin=2)
+    ds = read("degree_set")
+    assert list(ds[:, 0]) == sorted(c.sizes) and sorted(ds[:, 1]) == list(range(1, c.k + 1))
+    de = read("degree_element")
+    assert de.shape[0] == deg.size and list(de[:, 0]) == sorted(deg)
+    ec = read("element_cdf")
+    assert ec[-1, 1] == int((deg.astype(np.int64) ** 2).sum())
+    sc = read("set_cdf")
+    assert sc[-1, 1] == int(sum(deg[c.set(i)].sum() for i in range(c.k)))
