@@ -31,6 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# default SISI merge kernel (tile shape 0 of k_merge.hip): 256 threads x 31
+# elements per thread, 4096-element windows, materialising instantiation
+SISI_KERNEL = "k_sisi_stream<256,31,4096,materialise>"
 INT8_PEAK_TOPS = 5000.0
 METRIC = "SISI input GB/s vs HBM peak; MISI pairs/sec batched vs plain; exact counts"
 
@@ -143,8 +146,8 @@ def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
                        "intersection": int(exp.size), "bytes_per_step": int(nbytes)})
     rec["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                       "traffic": read_traffic(f"ip_sisi_materialize_2^{log2a}_pmc.md"),
-                       "kernel": "k_sisi_merge<256,16,materialise>", "kernel_ms": round(k_ms, 4),
+                       "traffic": read_traffic(f"k_sisi_stream_materialise_2^{log2a}x2^{log2b}_pmc.md"),
+                       "kernel": SISI_KERNEL, "kernel_ms": round(k_ms, 4),
                        "compact_kernel_ms": round(c_ms, 4)}
     rec["cpu_baseline"] = cpu
     rec["exact"] = exact

@@ -10,15 +10,20 @@
 //    whole pair (MISI).  Only the segment ends need a global co-rank search;
 //    it is done by half-waves with a 32-ary search (ceil(log32 n) dependent
 //    rounds instead of log2 n).
-//  * Inside the segment the workgroup streams tiles of T = NT*VT merged
-//    elements through two circular LDS windows (T elements of A, T of B).
-//    After each tile only the consumed elements are re-fetched, with aligned
-//    128-bit global loads issued BEFORE the tile's merge and committed to LDS
-//    after it, so HBM traffic is one read of each input element and the load
-//    latency overlaps the merge.
-//  * Per tile every thread finds its merge-path split by binary search in LDS
-//    at the END of its VT-element diagonal range; its start is its
-//    neighbour's end, so each thread does one search.
+//  * SISI (k_sisi_stream): the segment is streamed in tiles of up to
+//    T = NT*VT merged elements (VT odd, so that lanes reading about VT/2
+//    apart spread over the LDS banks) through two circular LDS windows of
+//    W ~ T/2 elements, which doubles the resident waves per CU over
+//    T-element windows.  A thread's split is kept only if it consumes at most
+//    W-1 elements of a side; feasible splits form a prefix of the threads and
+//    the tile is cut after them (for random data almost never).  Refill loads
+//    (aligned 128-bit, one read of every input element) are issued before the
+//    tile's merge and committed to LDS after it.
+//  * Per tile every thread finds its split by a branch-free binary search in
+//    LDS at the END of its diagonal range; its start is its neighbour's end.
+//    The merge loop re-reads both heads each step (two independent LDS reads,
+//    five VALU), and in count mode matches are counted per wave with
+//    ballot + scalar popcount.
 //  * Ties put the A element first, so an A element is matched against the B
 //    element that is next in line when it is merged; no partition can split a
 //    matching pair and no "count augmentation" (helpers.cuh:121-123) is needed.
@@ -40,13 +45,13 @@ using namespace si_dev;
 //   partials[s]     per-segment count (MAT) / partial sum (count mode)
 //   done            per-launch ticket counter, left at zero
 //   out             final uint64 count (count mode)
-template <int NT, int VT, bool MAT, bool DUAL = false>
+template <int NT, int VT, bool MAT>
 __global__ __launch_bounds__(NT) void k_sisi_merge(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
                                                    uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
                                                    uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
                                                    uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
                                                    uint32_t* __restrict__ stage, uint64_t stage_cap) {
-    __shared__ RingSmem<NT, VT> sm;
+    __shared__ WinSmem<NT, VT> sm;
     const uint32_t s = blockIdx.x;
     const uint64_t n = nA + nB;
     const uint64_t dBeg = uint64_t(s) * seg_len;
@@ -62,13 +67,61 @@ __global__ __launch_bounds__(NT) void k_sisi_merge(const uint32_t* __restrict__ 
     const uint64_t ia = sm.seg[0], iaE = sm.seg[1];
     const uint64_t ib = dBeg - ia, ibE = dEnd - iaE;
 
-    uint64_t c = merge_segment_pf<NT, VT, MAT, DUAL>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm,
-                                            MAT ? stage + uint64_t(s) * stage_cap : nullptr);
+    uint64_t c = merge_segment_win<NT, VT, MAT>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm,
+                                                MAT ? stage + uint64_t(s) * stage_cap : nullptr);
     const uint64_t total = block_sum_u64<NT>(c, sm.red);
     if (threadIdx.x == 0) partials[s] = total;
     if (MAT) return;
 
     // last workgroup folds the partials
+    __shared__ uint32_t isLast;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t ticket = atomicAdd(done, 1u);
+        isLast = (ticket == nseg - 1);
+    }
+    __syncthreads();
+    if (!isLast) return;
+    __threadfence();
+    uint64_t acc = 0;
+    for (uint32_t i = threadIdx.x; i < nseg; i += NT) acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc = block_sum_u64<NT>(acc, sm.red);
+    if (threadIdx.x == 0) {
+        if (out) *out = acc;
+        if (out32) *out32 = uint32_t(acc);
+        *done = 0;
+    }
+}
+
+// Same contract as k_sisi_merge, with half windows (merge_segment_stream).
+template <int NT, int VT, int WN, bool MAT, bool SEL>
+__global__ __launch_bounds__(NT) void k_sisi_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
+                                                    uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
+                                                    uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
+                                                    uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
+                                                    uint32_t* __restrict__ stage, uint64_t stage_cap) {
+    __shared__ StreamSmem<NT, VT, WN> sm;
+    const uint32_t s = blockIdx.x;
+    const uint64_t n = nA + nB;
+    const uint64_t dBeg = uint64_t(s) * seg_len;
+    const uint64_t dEnd = (dBeg + seg_len < n) ? dBeg + seg_len : n;
+
+    if (threadIdx.x < WAVE) {
+        const uint64_t dd = (lane_id() < 32) ? dBeg : dEnd;
+        const uint64_t c = corank_halfwave(base + oA, nA, base + oB, nB, dd);
+        if (lane_id() == 0) sm.seg[0] = c;
+        if (lane_id() == 32) sm.seg[1] = c;
+    }
+    __syncthreads();
+    const uint64_t ia = sm.seg[0], iaE = sm.seg[1];
+    const uint64_t ib = dBeg - ia, ibE = dEnd - iaE;
+
+    uint64_t c = merge_segment_stream<NT, VT, WN, MAT, SEL>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm,
+                                                       MAT ? stage + uint64_t(s) * stage_cap : nullptr);
+    const uint64_t total = block_sum_u64<NT>(c, sm.red);
+    if (threadIdx.x == 0) partials[s] = total;
+    if (MAT) return;
+
     __shared__ uint32_t isLast;
     if (threadIdx.x == 0) {
         __threadfence();
@@ -144,13 +197,15 @@ constexpr int RUN_VT = 8;
 namespace si_impl {
 
 // Tile shapes of the SISI merge kernel (threads x elements per thread) and
-// resident workgroups per CU; shape 0 is the default, the others are
-// selectable through si_params.reserved[0] for tuning sweeps.
+// workgroups per CU; shape 0 is the default, the others are selectable
+// through si_params.reserved[0] (count) or SI_SISI_SHAPE (materialisation)
+// for tuning sweeps.  Shapes 0-5 are k_sisi_stream (half windows; 1 reads
+// one head per step), 6-7 k_sisi_merge (T-element windows).
 struct SisiShape {
     int nt, vt, per_cu;
 };
-static const SisiShape kShapes[] = {{256, 8, 4}, {256, 8, 4}, {512, 8, 2}, {128, 16, 4},
-                                     {128, 8, 8}, {256, 16, 2}, {128, 8, 8}};  // 1, 6: dual-read merge loop
+static const SisiShape kShapes[] = {{256, 31, 4}, {256, 31, 4}, {128, 31, 8}, {256, 15, 8},
+                                    {128, 15, 16}, {512, 15, 4}, {256, 15, 4}, {512, 15, 4}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, const SisiShape& sh, uint32_t& nseg,
@@ -167,27 +222,38 @@ static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, const
     if (nseg == 0) nseg = 1;
 }
 
-template <int NT, int VT, bool MAT, bool DUAL = false>
+template <int NT, int VT, bool MAT>
 static void launch_sisi(si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA, uint64_t oB,
                         uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done, uint64_t* out,
                         uint32_t* out32, uint32_t* stage, uint64_t cap) {
-    hipLaunchKernelGGL((k_sisi_merge<NT, VT, MAT, DUAL>), dim3(nseg), dim3(NT), 0, ctx->stream, base, oA, nA, oB, nB, seg_len,
-                       nseg, partials, done, out, out32, stage, cap);
+    hipLaunchKernelGGL((k_sisi_merge<NT, VT, MAT>), dim3(nseg), dim3(NT), 0, ctx->stream, base, oA, nA, oB, nB,
+                       seg_len, nseg, partials, done, out, out32, stage, cap);
+}
+
+template <int NT, int VT, int WN, bool SEL, bool MAT>
+static void launch_stream(si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA,
+                          uint64_t oB, uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done,
+                          uint64_t* out, uint32_t* out32, uint32_t* stage, uint64_t cap) {
+    hipLaunchKernelGGL((k_sisi_stream<NT, VT, WN, MAT, SEL>), dim3(nseg), dim3(NT), 0, ctx->stream, base, oA, nA, oB, nB,
+                       seg_len, nseg, partials, done, out, out32, stage, cap);
 }
 
 template <bool MAT>
 static void launch_shape(int shape, si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA,
                          uint64_t oB, uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done, uint64_t* out,
                          uint32_t* out32, uint32_t* stage, uint64_t cap) {
+#define SI_SISI_ARGS ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap
     switch (shape) {
-        case 1: launch_sisi<256, 8, MAT, true>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
-        case 2: launch_sisi<512, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
-        case 3: launch_sisi<128, 16, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
-        case 4: launch_sisi<128, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
-        case 5: launch_sisi<256, 16, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
-        case 6: launch_sisi<128, 8, MAT, true>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
-        default: launch_sisi<256, 8, MAT>(ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap); break;
+        case 1: launch_stream<256, 31, 4096, true, MAT>(SI_SISI_ARGS); break;
+        case 2: launch_stream<128, 31, 2048, false, MAT>(SI_SISI_ARGS); break;
+        case 3: launch_stream<256, 15, 2048, false, MAT>(SI_SISI_ARGS); break;
+        case 4: launch_stream<128, 15, 1024, false, MAT>(SI_SISI_ARGS); break;
+        case 5: launch_stream<512, 15, 4096, false, MAT>(SI_SISI_ARGS); break;
+        case 6: launch_sisi<256, 15, MAT>(SI_SISI_ARGS); break;
+        case 7: launch_sisi<512, 15, MAT>(SI_SISI_ARGS); break;
+        default: launch_stream<256, 31, 4096, false, MAT>(SI_SISI_ARGS); break;
     }
+#undef SI_SISI_ARGS
 }
 
 static int shape_of(const si_params* p) {

@@ -296,41 +296,45 @@ __device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA
 }
 
 // --------------------------------------------------------------------------
-// Prefetching variant for long segments (SISI).  Each side keeps a ring of
-// R = 2T elements in LDS: while tile k is merged, the ring already holds the
-// next T elements past tile k's window, and the loads that replace tile k's
-// consumed elements are issued during tile k but committed only at the end
-// of tile k+1 (two register sets), so every global load has a whole extra
-// tile of time to arrive.
-//   ring content at the start of tile k: A[ia_k, ia_k + 2T) (clipped to |A|)
-//   tile k issues A[ia_k + 2T, ia_{k+1} + 2T) (= ca_k elements)
-//   tile k commits tile k-1's loads into the slots consumed by tile k-1.
+// Streaming merge for long segments (SISI), for any VT.  Each side keeps a
+// circular window indexed modulo W, the power of two >= T = NT*VT; the window
+// holds A[ia, ia + T) (B[ib, ib + T)) and after a tile consuming ca elements
+// of A the positions [ia + T, ia + T + ca) are fetched into the freed slots.
+// Lanes read the windows about VT/2 words apart per side: an even VT puts
+// whole groups of lanes on the same LDS bank, an odd VT spreads them over all
+// banks, which is why the tile need not be a power of two.
+constexpr uint32_t pow2_ceil(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
 template <int NT, int VT>
-struct RingCfg {
-    static constexpr int T = NT * VT;
-    static constexpr uint32_t R = 2 * T;      // ring capacity per side
-    static constexpr uint32_t M = R - 1;
-    static constexpr int CPT = (T / 4 + 4 + NT - 1) / NT;
-    static_assert((T & (T - 1)) == 0, "tile must be a power of two");
+struct WinCfg {
+    static constexpr uint32_t T = uint32_t(NT) * VT;   // merged elements per tile
+    static constexpr uint32_t W = pow2_ceil(T);        // window modulus per side
+    static constexpr uint32_t M = W - 1;
+    static constexpr int CPT = int((T / 4 + 4 + NT - 1) / NT);  // 16-byte chunks per thread per refill
 };
 
 template <int NT, int VT>
-struct RingSmem {
-    uint32_t ring[2 * 2 * NT * VT];  // A ring [0, R), B ring [R, 2R)
+struct WinSmem {
+    uint32_t win[2 * WinCfg<NT, VT>::W];  // A window [0, W), B window [W, 2W)
     uint32_t ends[NT];
     uint32_t wave_tot[NT / WAVE];
     uint64_t red[NT / WAVE];
     uint64_t seg[4];
 };
 
-// A refill of at most T elements per side, held in registers.
+// Loads of one refill (at most T elements over both sides), held in registers
+// between issue and commit.  Every lane issues all CPT loads back to back;
+// chunk numbers past the needed ones are clamped to the last needed chunk (in
+// bounds, not committed).
 template <int NT, int VT>
-struct RingLoad {
-    using C = RingCfg<NT, VT>;
+struct WinLoad {
+    using C = WinCfg<NT, VT>;
     uint4 v[C::CPT];
     uint32_t nCA, nCB, phA, phB, lenA, lenB, sA, sB;
-
-    __device__ __forceinline__ void none() { nCA = nCB = 0; }
 
     __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t a0, uint64_t a1, uint64_t b0,
                                           uint64_t b1) {
@@ -357,6 +361,8 @@ struct RingLoad {
         }
     }
 
+    // chunk c of a range holds elements rel = 4c - ph + q, needed for
+    // 0 <= rel < len; an aligned chunk stays contiguous in the window (W % 4 == 0)
     __device__ __forceinline__ static void put(uint32_t* w, const uint4& x, uint32_t c, uint32_t ph, uint32_t len,
                                                uint32_t s0) {
         const int rel = int(4 * c) - int(ph);
@@ -371,24 +377,28 @@ struct RingLoad {
         }
     }
 
-    __device__ __forceinline__ void commit(RingSmem<NT, VT>& sm) const {
+    __device__ __forceinline__ void commit(WinSmem<NT, VT>& sm) const {
 #pragma unroll
         for (int j = 0; j < C::CPT; ++j) {
             const uint32_t c = threadIdx.x + uint32_t(j) * NT;
             if (c < nCA)
-                put(sm.ring, v[j], c, phA, lenA, sA);
+                put(sm.win, v[j], c, phA, lenA, sA);
             else if (c < nCA + nCB)
-                put(sm.ring + C::R, v[j], c - nCA, phB, lenB, sB);
+                put(sm.win + C::W, v[j], c - nCA, phB, lenB, sB);
         }
     }
 };
 
-template <int NT, int VT, bool MAT, bool DUAL>
-__device__ uint64_t merge_segment_pf(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
-                                     uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
-                                     RingSmem<NT, VT>& sm, uint32_t* __restrict__ stage) {
-    using C = RingCfg<NT, VT>;
+// Streams the merge path from (ia, ib) to (iaEnd, ibEnd) of sets A (global
+// element offset oA, size nA) and B (oB, nB).  Returns this thread's match
+// count; with MAT the workgroup's matches are written in order to stage[0 ..).
+template <int NT, int VT, bool MAT>
+__device__ uint64_t merge_segment_win(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
+                                      uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
+                                      WinSmem<NT, VT>& sm, uint32_t* __restrict__ stage) {
+    using C = WinCfg<NT, VT>;
     constexpr uint32_t T = C::T;
+    constexpr uint32_t M = C::M;
     const uint32_t tid = threadIdx.x;
     uint64_t myCount = 0;
     uint64_t written = 0;
@@ -397,35 +407,27 @@ __device__ uint64_t merge_segment_pf(const uint32_t* __restrict__ base, uint64_t
     const uint64_t dEnd = iaEnd + ibEnd;
     if (d >= dEnd) return 0;
 
-    // initial fill: A[ia, ia + 2T) and B[ib, ib + 2T), in four parts of <= T
     {
-        RingLoad<NT, VT> r;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const uint64_t a0 = ia + uint64_t(h) * T < nA ? ia + uint64_t(h) * T : nA;
-            const uint64_t a1 = ia + uint64_t(h + 1) * T < nA ? ia + uint64_t(h + 1) * T : nA;
-            r.issue(base, oA + a0, oA + a1, 0, 0);
-            r.commit(sm);
-            const uint64_t b0 = ib + uint64_t(h) * T < nB ? ib + uint64_t(h) * T : nB;
-            const uint64_t b1 = ib + uint64_t(h + 1) * T < nB ? ib + uint64_t(h + 1) * T : nB;
-            r.issue(base, 0, 0, oB + b0, oB + b1);
-            r.commit(sm);
-        }
+        WinLoad<NT, VT> r;
+        const uint64_t a1 = ia + T < nA ? ia + T : nA;
+        r.issue(base, oA + ia, oA + a1, 0, 0);
+        r.commit(sm);
+        const uint64_t b1 = ib + T < nB ? ib + T : nB;
+        r.issue(base, 0, 0, oB + ib, oB + b1);
+        r.commit(sm);
     }
     __syncthreads();
 
-    RingLoad<NT, VT> pend;
-    pend.none();
+    const uint32_t* rA = sm.win;
+    const uint32_t* rB = sm.win + C::W;
     while (d < dEnd) {
         const uint32_t Tc = uint32_t((dEnd - d) < T ? (dEnd - d) : T);
         const uint32_t nAw = uint32_t((nA - ia) < T ? (nA - ia) : T);
         const uint32_t nBw = uint32_t((nB - ib) < T ? (nB - ib) : T);
-        const uint32_t sa = uint32_t(oA + ia) & C::M;  // ring slot of A[ia]
-        const uint32_t sb = uint32_t(oB + ib) & C::M;
-        const uint32_t* rA = sm.ring;
-        const uint32_t* rB = sm.ring + C::R;
-        auto WA = [&](uint32_t i) { return rA[(sa + i) & C::M]; };
-        auto WB = [&](uint32_t i) { return rB[(sb + i) & C::M]; };
+        const uint32_t sa = uint32_t(oA + ia) & M;  // window slot of A[ia]
+        const uint32_t sb = uint32_t(oB + ib) & M;
+        auto WA = [&](uint32_t i) { return rA[(sa + i) & M]; };
+        auto WB = [&](uint32_t i) { return rB[(sb + i) & M]; };
         const bool full = (Tc == T) && (nAw == T) && (nBw == T);
 
         // 1. split at the end of this thread's diagonal range
@@ -451,57 +453,43 @@ __device__ uint64_t merge_segment_pf(const uint32_t* __restrict__ base, uint64_t
         const uint32_t ca = sm.ends[NT - 1];
         const uint32_t cb = Tc - ca;
 
-        // 2. issue the loads that replace this tile's consumed elements, 2T ahead
-        RingLoad<NT, VT> nxt;
+        // 2. fetch the elements that replace the consumed ones (registers)
+        WinLoad<NT, VT> r;
         {
-            const uint64_t a0 = ia + 2 * T < nA ? ia + 2 * T : nA;
-            const uint64_t a1 = ia + 2 * T + ca < nA ? ia + 2 * T + ca : nA;
-            const uint64_t b0 = ib + 2 * T < nB ? ib + 2 * T : nB;
-            const uint64_t b1 = ib + 2 * T + cb < nB ? ib + 2 * T + cb : nB;
-            nxt.issue(base, oA + a0, oA + a1, oB + b0, oB + b1);
+            const uint64_t a0 = ia + T < nA ? ia + T : nA;
+            const uint64_t a1 = ia + T + ca < nA ? ia + T + ca : nA;
+            const uint64_t b0 = ib + T < nB ? ib + T : nB;
+            const uint64_t b1 = ib + T + cb < nB ? ib + T + cb : nB;
+            r.issue(base, oA + a0, oA + a1, oB + b0, oB + b1);
         }
 
-        // 3. merge this thread's VT steps (A first on ties; a tie is a match)
+        // 3. merge this thread's VT steps; A is taken on ties and matched
+        //    against the current B head (which may lie past y1)
         uint32_t cnt = 0;
         uint32_t mval[MAT ? VT : 1];
         uint32_t mbits = 0;
         if (full) {
-            if (DUAL) {
-                uint32_t x = x0;
-                const uint32_t cbase = sb + x0 + y0;  // B slot of step k: (cbase + k - x) & M
-                uint32_t av = rA[(sa + x) & C::M], bv = rB[(cbase - x) & C::M];
+            // Every thread has exactly VT steps and the heads A[x], B[y] are
+            // in the windows for x <= x1, y <= y1 (x1 == T or y1 == T only
+            // when the other side has no steps), so the comparison alone
+            // decides: take A iff A[x] <= B[y]; a tie is a match.
+            uint32_t pa = sa + x0, pb = C::W + sb + y0;  // unmasked positions
+            uint32_t av = rA[pa & M], bv = sm.win[C::W + ((pb - C::W) & M)];
 #pragma unroll
-                for (int k = 0; k < VT; ++k) {
-                    const bool takeA = av <= bv;
-                    const bool hit = av == bv;
-                    cnt += hit;
-                    if (MAT) {
-                        mval[k] = av;
-                        mbits |= uint32_t(hit) << k;
-                    }
-                    x += takeA;
-                    av = rA[(sa + x) & C::M];
-                    bv = rB[(cbase + k + 1 - x) & C::M];
+            for (int k = 0; k < VT; ++k) {
+                const bool takeA = av <= bv;
+                const bool hit = av == bv;
+                cnt += hit;
+                if (MAT) {
+                    mval[k] = av;
+                    mbits |= uint32_t(hit) << k;
                 }
-            } else {
-                uint32_t pa = sa + x0, pb = C::R + sb + y0;  // unmasked ring positions
-                uint32_t av = WA(x0), bv = WB(y0);
-#pragma unroll
-                for (int k = 0; k < VT; ++k) {
-                    const bool takeA = av <= bv;
-                    const bool hit = av == bv;
-                    cnt += hit;
-                    if (MAT) {
-                        mval[k] = av;
-                        mbits |= uint32_t(hit) << k;
-                    }
-                    pa += takeA;
-                    pb += !takeA;
-                    const uint32_t slot = takeA ? (pa & C::M) : (C::R + ((pb - C::R) & C::M));
-                    const uint32_t nv = sm.ring[slot];
-                    av = takeA ? nv : av;
-                    bv = takeA ? bv : nv;
-                }
+                pa += takeA;
+                pb += !takeA;
+                const uint32_t slot = takeA ? (pa & M) : (C::W + ((pb - C::W) & M));
+                const uint32_t nv = sm.win[slot];
+                av = takeA ? nv : av;
+                bv = takeA ? bv : nv;
             }
         } else {
             uint32_t x = x0, y = y0;
@@ -536,11 +524,10 @@ __device__ uint64_t merge_segment_pf(const uint32_t* __restrict__ base, uint64_t
             excl = incl - cnt;
             if (lane_id() == WAVE - 1) sm.wave_tot[tid / WAVE] = incl;
         }
-        __syncthreads();  // all ring reads of this tile are done
+        __syncthreads();  // all window reads of this tile are done
 
-        // 4. commit the previous tile's loads; with MAT, stage this tile's matches
-        pend.commit(sm);
-        pend = nxt;
+        // 4. commit the refill; with MAT, stage this tile's matches
+        r.commit(sm);
         if (MAT) {
             const uint32_t w = tid / WAVE;
             uint32_t before = 0;
@@ -554,6 +541,323 @@ __device__ uint64_t merge_segment_pf(const uint32_t* __restrict__ base, uint64_t
 #pragma unroll
             for (int k = 0; k < VT; ++k) {
                 if (mbits & (1u << k)) *dst++ = mval[k];
+            }
+            written += wtot;
+        }
+        ia += ca;
+        ib += cb;
+        d += Tc;
+        __syncthreads();
+    }
+    return myCount;
+}
+
+// --------------------------------------------------------------------------
+// Half-window streaming merge (SISI).  A tile of up to T = NT*VT merged
+// elements consumes about T/2 from each side, so each side keeps only a window
+// of W elements (a power of two a little above T/2): the window holds
+// A[ia, ia + W) at slot (absolute index & (W-1)), and the first PAD slots are
+// mirrored past the end so that a thread's VT+1 consecutive reads need no
+// masking.  A thread's split is feasible when it consumes at most capA
+// elements of A and capB of B (W-1 while at least W remain, else all that
+// remain); feasibility is monotone along the diagonal, so the
+// feasible threads form a prefix and the tile is cut after them.  For random
+// data a tile is cut only when one side runs more than (W - T/2) ahead.
+template <int NT, int VT, int WN>
+struct StreamCfg {
+    static constexpr uint32_t T = uint32_t(NT) * VT;
+    static constexpr uint32_t W = WN;
+    static constexpr uint32_t M = W - 1;
+    static constexpr uint32_t PAD = (uint32_t(VT) + 2 + 3) & ~3u;  // mirrored head slots
+    static constexpr uint32_t SPAN = W + PAD;                       // LDS words per side
+    static constexpr int CPT = int((T / 4 + 4 + NT - 1) / NT);      // 16-byte chunks per thread per refill
+    static_assert((W & (W - 1)) == 0, "window must be a power of two");
+    static_assert(uint32_t(VT) + 1 < W, "window shorter than a thread's range");
+    static_assert(W / 4 + 2 <= uint32_t(CPT) * NT, "initial fill exceeds one load");
+    static_assert(VT <= 32, "match bits are kept in one 32-bit word");
+};
+
+template <int NT, int VT, int WN>
+struct StreamSmem {
+    alignas(16) uint32_t win[2 * StreamCfg<NT, VT, WN>::SPAN];  // A window [0, SPAN), B window [SPAN, 2 SPAN)
+    uint32_t ends[NT];
+    uint32_t wave_feas[NT / WAVE];
+    uint32_t wave_tot[NT / WAVE];
+    uint64_t red[NT / WAVE];
+    uint64_t seg[4];
+};
+
+// One refill of at most T elements over both sides, held in registers from
+// issue to commit.  Every lane issues all CPT loads back to back; chunk
+// numbers past the needed ones are clamped to the last needed chunk (in
+// bounds, not committed).
+template <int NT, int VT, int WN>
+struct StreamLoad {
+    using C = StreamCfg<NT, VT, WN>;
+    uint4 v[C::CPT];
+    uint32_t nCA, nCB, phA, phB, lenA, lenB, sA, sB;
+
+    __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t a0, uint64_t a1, uint64_t b0,
+                                          uint64_t b1) {
+        const uint4* b4 = reinterpret_cast<const uint4*>(base);
+        lenA = a1 > a0 ? uint32_t(a1 - a0) : 0u;
+        lenB = b1 > b0 ? uint32_t(b1 - b0) : 0u;
+        nCA = lenA ? uint32_t(((a1 + 3) >> 2) - (a0 >> 2)) : 0u;
+        nCB = lenB ? uint32_t(((b1 + 3) >> 2) - (b0 >> 2)) : 0u;
+        phA = uint32_t(a0 & 3);
+        phB = uint32_t(b0 & 3);
+        sA = uint32_t(a0 & ~3ull) & C::M;
+        sB = uint32_t(b0 & ~3ull) & C::M;
+        const uint4* pA = b4 + (a0 >> 2);
+        const uint4* pB = b4 + (b0 >> 2);
+        const uint32_t tot = nCA + nCB;
+        if (tot) {
+#pragma unroll
+            for (int j = 0; j < C::CPT; ++j) {
+                uint32_t c = threadIdx.x + uint32_t(j) * NT;
+                c = c < tot ? c : tot - 1;
+                const uint4* p = c < nCA ? pA + c : pB + (c - nCA);
+                v[j] = *p;
+            }
+        }
+    }
+
+    // chunk c of a range holds elements rel = 4c - ph + q, needed for
+    // 0 <= rel < len; slots below PAD are written to their mirror as well
+    __device__ __forceinline__ static void put(uint32_t* w, const uint4& x, uint32_t c, uint32_t ph, uint32_t len,
+                                               uint32_t s0) {
+        const int rel = int(4 * c) - int(ph);
+        const uint32_t slot = (s0 + 4 * c) & C::M;
+        if (rel >= 0 && rel + 4 <= int(len)) {
+            *reinterpret_cast<uint4*>(&w[slot]) = x;
+            if (slot < C::PAD) *reinterpret_cast<uint4*>(&w[C::W + slot]) = x;
+        } else {
+            const uint32_t e[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (rel + q >= 0 && rel + q < int(len)) {
+                    w[slot + q] = e[q];
+                    if (slot < C::PAD) w[C::W + slot + q] = e[q];
+                }
+        }
+    }
+
+    __device__ __forceinline__ void commit(StreamSmem<NT, VT, WN>& sm) const {
+#pragma unroll
+        for (int j = 0; j < C::CPT; ++j) {
+            const uint32_t c = threadIdx.x + uint32_t(j) * NT;
+            if (c < nCA)
+                put(sm.win, v[j], c, phA, lenA, sA);
+            else if (c < nCA + nCB)
+                put(sm.win + C::SPAN, v[j], c - nCA, phB, lenB, sB);
+        }
+    }
+};
+
+// First i in [lo, hi] such that A[i] > B[dd - 1 - i] (hi if none), for
+// hi - lo <= 2*STEP - 1, with STEP a power of two; A[i] is at
+// wa[(sa + i) & M], B[j] at wb[(sb + j) & M].
+// Branch-free power-of-two stepping on byte offsets: both reads are issued
+// every step and out-of-range candidates are discarded.
+template <uint32_t M, uint32_t STEP>
+__device__ __forceinline__ uint32_t split_search(const uint32_t* wa, uint32_t sa, const uint32_t* wb, uint32_t sb,
+                                                 uint32_t dd, uint32_t lo, uint32_t hi) {
+    const char* pa = reinterpret_cast<const char*>(wa);
+    const char* pb = reinterpret_cast<const char*>(wb);
+    constexpr uint32_t M4 = M * 4u;
+    const uint32_t am1 = (sa - 1u) * 4u;  // A[c - 1] at byte (am1 + 4c) & M4
+    const uint32_t bk = (sb + dd) * 4u;   // B[dd - c] at byte (bk - 4c) & M4
+    const uint32_t hi4 = hi * 4u;
+    uint32_t x4 = lo * 4u;
+#pragma unroll
+    for (uint32_t s = STEP; s; s >>= 1) {
+        const uint32_t c4 = x4 + 4u * s;
+        const uint32_t av = *reinterpret_cast<const uint32_t*>(pa + ((am1 + c4) & M4));
+        const uint32_t bv = *reinterpret_cast<const uint32_t*>(pb + ((bk - c4) & M4));
+        const bool ok = (c4 <= hi4) & (av <= bv);
+        x4 = ok ? c4 : x4;
+    }
+    return x4 >> 2;
+}
+
+// SEL: one LDS read per merge step (the next element of the side just
+// consumed, selected by address) instead of re-reading both heads.
+template <int NT, int VT, int WN, bool MAT, bool SEL = false>
+__device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
+                                         uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
+                                         StreamSmem<NT, VT, WN>& sm, uint32_t* __restrict__ stage) {
+    using C = StreamCfg<NT, VT, WN>;
+    constexpr uint32_t T = C::T, W = C::W, M = C::M;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id();
+    uint64_t myCount = 0;
+    uint64_t written = 0;
+
+    uint64_t d = ia + ib;
+    const uint64_t dEnd = iaEnd + ibEnd;
+    if (d >= dEnd) return 0;
+
+    {
+        StreamLoad<NT, VT, WN> r;
+        const uint64_t a1 = ia + W < nA ? ia + W : nA;
+        const uint64_t b1 = ib + W < nB ? ib + W : nB;
+        r.issue(base, oA + ia, oA + a1, 0, 0);
+        r.commit(sm);
+        r.issue(base, 0, 0, oB + ib, oB + b1);
+        r.commit(sm);
+    }
+    __syncthreads();
+
+    const uint32_t* rA = sm.win;
+    const uint32_t* rB = sm.win + C::SPAN;
+    while (d < dEnd) {
+        const uint64_t rem64 = dEnd - d;
+        const uint32_t rem = uint32_t(rem64 < T ? rem64 : T);
+        // a side is cut when the window cannot hold all that remains of it;
+        // a tile then consumes at most W-1 of its elements, so every head the
+        // merge reads (index <= W-1) is a true element
+        const bool aMore = (nA - ia) >= W;
+        const bool bMore = (nB - ib) >= W;
+        const uint32_t capA = aMore ? W - 1 : uint32_t(nA - ia);
+        const uint32_t capB = bMore ? W - 1 : uint32_t(nB - ib);
+        const uint32_t sa = uint32_t(oA + ia) & M;  // window slot of A[ia]
+        const uint32_t sb = uint32_t(oB + ib) & M;
+
+        // 1. feasible split at the end of this thread's diagonal range
+        const uint32_t dd = (tid + 1) * VT < rem ? (tid + 1) * VT : rem;
+        {
+            const uint32_t lo = dd > capB ? dd - capB : 0u;
+            const uint32_t hi0 = dd < capA ? dd : capA;
+            bool feas = lo <= hi0;
+            const uint32_t hi = feas ? hi0 : lo;
+            const uint32_t r = split_search<M, W / 2>(rA, sa, rB, sb, dd, lo, hi);  // hi - lo <= W - 1
+            // the true split may lie left of lo (more than capB from B) only
+            // when B continues, or right of hi (more than capA from A) only
+            // when A continues: test the predicate just outside
+            if (r == lo && lo > 0 && bMore) feas = feas && !(rA[(sa + lo - 1) & M] > rB[(sb + dd - lo) & M]);
+            if (r == hi && hi < dd && aMore) feas = feas && (rA[(sa + hi) & M] > rB[(sb + dd - 1 - hi) & M]);
+            sm.ends[tid] = r;
+            const uint64_t fb = __ballot(feas);
+            if (lane == 0) sm.wave_feas[tid / WAVE] = uint32_t(__popcll(fb));
+        }
+        __syncthreads();
+        uint32_t F = 0;
+#pragma unroll
+        for (int i = 0; i < NT / WAVE; ++i) F += sm.wave_feas[i];
+        const uint32_t Tc = F * VT < rem ? F * VT : rem;
+        const uint32_t ca = sm.ends[F - 1];
+        const uint32_t cb = Tc - ca;
+        const bool active = tid < F;
+        const uint32_t ds = active ? (tid * VT < Tc ? tid * VT : Tc) : Tc;
+        const uint32_t de = active ? dd : Tc;
+        const uint32_t x0 = active ? (tid ? sm.ends[tid - 1] : 0u) : ca;
+        const uint32_t x1 = active ? sm.ends[tid] : ca;
+        const uint32_t y0 = ds - x0;
+        const uint32_t y1 = de - x1;
+
+        // 2. fetch the elements that replace the consumed ones (registers)
+        StreamLoad<NT, VT, WN> r;
+        {
+            const uint64_t a0 = ia + W < nA ? ia + W : nA;
+            const uint64_t a1 = ia + W + ca < nA ? ia + W + ca : nA;
+            const uint64_t b0 = ib + W < nB ? ib + W : nB;
+            const uint64_t b1 = ib + W + cb < nB ? ib + W + cb : nB;
+            r.issue(base, oA + a0, oA + a1, oB + b0, oB + b1);
+        }
+
+        // 3. merge this thread's steps; A is taken on ties and matched against
+        //    the current B head (which may lie past y1)
+        uint32_t cnt = 0;
+        uint32_t mval[MAT ? VT : 1];
+        uint32_t mbits = 0;  // MAT: bit (VT-1-k) set when step k is a match
+        const uint32_t steps = de - ds;
+        if (aMore && bMore && steps == uint32_t(VT)) {
+            // x1 <= W-1 and y1 <= W-1: every head read is a true element and
+            // a thread's reads stay within VT+1 slots of its (masked) start.
+            // A[x] at byte pa + 4x, B[ds + k - x] at byte pb + 4k - 4x
+            const char* pa = reinterpret_cast<const char*>(rA + (((sa + x0) & M) - x0));
+            const char* pb = reinterpret_cast<const char*>(rB + (((sb + y0) & M) - y0 + ds));
+            uint32_t x = x0;
+            uint32_t av = *reinterpret_cast<const uint32_t*>(pa + 4 * x);
+            uint32_t bv = *reinterpret_cast<const uint32_t*>(pb - 4 * x);
+            uint32_t hits = 0;
+#pragma unroll
+            for (int k = 0; k < VT; ++k) {
+                const bool takeA = av <= bv;
+                const bool hit = av == bv;
+                if (MAT) {
+                    mval[k] = av;
+                    mbits = 2 * mbits + uint32_t(hit);
+                } else {
+                    hits += uint32_t(__popcll(__ballot(hit)));
+                }
+                x += uint32_t(takeA);
+                // 24-bit multiply-add keeps the B address one instruction
+                const char* qa = pa + 4 * x;
+                const char* qb = pb + __mul24(int(x), -4) + 4 * (k + 1);
+                if (SEL) {
+                    const uint32_t nv = *reinterpret_cast<const uint32_t*>(takeA ? qa : qb);
+                    av = takeA ? nv : av;
+                    bv = takeA ? bv : nv;
+                } else {
+                    av = *reinterpret_cast<const uint32_t*>(qa);
+                    bv = *reinterpret_cast<const uint32_t*>(qb);
+                }
+            }
+            if (MAT)
+                cnt = uint32_t(__popc(mbits));
+            else if (lane == uint32_t(__ffsll(__ballot(1)) - 1))
+                cnt = hits;
+        } else if (steps) {
+            const uint32_t nAw = uint32_t((nA - ia) < W ? (nA - ia) : W);
+            const uint32_t nBw = uint32_t((nB - ib) < W ? (nB - ib) : W);
+            uint32_t x = x0, y = y0;
+            uint32_t av = x < x1 ? rA[(sa + x) & M] : 0u;
+            uint32_t bv = y < nBw ? rB[(sb + y) & M] : 0u;
+#pragma unroll
+            for (int k = 0; k < VT; ++k) {
+                bool hit = false;
+                if (uint32_t(k) < steps) {
+                    const bool takeA = (x < x1) && (y >= y1 || av <= bv);
+                    hit = takeA && (y < nBw) && (av == bv);
+                    if (MAT) mval[k] = av;
+                    if (takeA) {
+                        ++x;
+                        if (x < x1) av = rA[(sa + x) & M];
+                    } else {
+                        ++y;
+                        if (y < nBw) bv = rB[(sb + y) & M];
+                    }
+                }
+                cnt += uint32_t(hit);
+                if (MAT) mbits = 2 * mbits + uint32_t(hit);
+            }
+        }
+        myCount += cnt;
+
+        uint32_t excl = 0, wtot = 0;
+        if (MAT) {
+            const uint32_t incl = wave_inclusive_scan_u32(cnt);
+            excl = incl - cnt;
+            if (lane == WAVE - 1) sm.wave_tot[tid / WAVE] = incl;
+        }
+        __syncthreads();  // all window reads of this tile are done
+
+        // 4. commit the refill; with MAT, stage this tile's matches
+        r.commit(sm);
+        if (MAT) {
+            const uint32_t w = tid / WAVE;
+            uint32_t before = 0;
+#pragma unroll
+            for (int i = 0; i < NT / WAVE; ++i) {
+                const uint32_t t = sm.wave_tot[i];
+                if (i < int(w)) before += t;
+                wtot += t;
+            }
+            uint32_t* dst = stage + written + before + excl;
+#pragma unroll
+            for (int k = 0; k < VT; ++k) {
+                if (mbits & (1u << (VT - 1 - k))) *dst++ = mval[k];
             }
             written += wtot;
         }

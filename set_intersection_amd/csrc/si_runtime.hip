@@ -1,6 +1,7 @@
 // libsi.so runtime: contexts, device memory, datasets, event timing, and the
 // C-ABI entry points that dispatch to the technique launchers.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -185,6 +186,8 @@ int si_open(int device, si_context** out) {
         return fail(SI_ERR_HIP, std::string("si_open: ") + hipGetErrorString(e));
     }
     ctx->num_cus = ctx->prop.multiProcessorCount;
+    // tile shape of the SISI merge kernel for materialisation (tuning sweeps)
+    if (const char* e = std::getenv("SI_SISI_SHAPE")) ctx->sisi_shape = std::atoi(e);
     *out = ctx;
     return SI_OK;
 }

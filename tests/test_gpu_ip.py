@@ -150,3 +150,52 @@ def test_runs_random_collection(ctx):
     k = coll.k
     for (i, j), c in got.items():
         assert c == counts[oracle.triangular_index(k, i, j)]
+
+
+def _adversarial_pairs():
+    """Pairs whose merge path runs far ahead on one side (tiles cut short),
+    ends one set mid-tile, or matches on every step."""
+    r = np.random.default_rng(5)
+    runs_a, runs_b, v = [], [], 0
+    for i in range(60):
+        la, lb = int(r.integers(1, 9000)), int(r.integers(1, 9000))
+        runs_a.append(np.arange(v, v + la))
+        v += la
+        runs_b.append(np.arange(v - (i % 3), v - (i % 3) + lb))  # seams share 0-2 values
+        v += lb
+    yield np.unique(np.concatenate(runs_a)).astype(np.uint32), np.unique(np.concatenate(runs_b)).astype(np.uint32)
+    big = datasets.uniform_sorted(1_000_000, 1 << 32, 3)
+    yield big, big[::997].copy()
+    yield big, big
+    yield big[:500_000].copy(), big[250_000:].copy()
+    yield big[::2].copy(), big[1::2].copy()
+    yield np.arange(0, 3_000_000, 3, dtype=np.uint32), np.arange(0, 3_000_000, 5, dtype=np.uint32)
+
+
+SISI_SHAPES = 8  # tile shapes of the SISI merge kernel (si_params.reserved[0])
+
+
+@pytest.mark.parametrize("shape", range(SISI_SHAPES))
+def test_tile_shapes(shape, monkeypatch):
+    """Every SISI tile shape (count: si_params.reserved[0]; materialise:
+    SI_SISI_SHAPE at context open) is exact on adversarial pairs."""
+    monkeypatch.setenv("SI_SISI_SHAPE", str(shape))
+    c = si.Context(0)
+    try:
+        p = si.params()
+        p.reserved[0] = shape
+        for a, b in _adversarial_pairs():
+            coll = si.Collection.from_sets([a, b])
+            ds = c.upload(coll)
+            exp = oracle.materialize(coll.set(0), coll.set(1))
+            assert ds.pair_count(0, 1, p=p) == exp.size, (shape, a.size, b.size)
+            assert ds.pair_count(1, 0, p=p) == exp.size, (shape, a.size, b.size)
+            for blocks in (1, 7, 300):
+                q = si.params(blocks=blocks)
+                q.reserved[0] = shape
+                assert ds.pair_count(0, 1, p=q) == exp.size, (shape, blocks)
+            assert np.array_equal(ds.pair_materialize(0, 1), exp), (shape, a.size, b.size)
+            assert np.array_equal(ds.pair_materialize(1, 0), exp), (shape, a.size, b.size)
+            ds.free()
+    finally:
+        c.close()
