@@ -3,8 +3,8 @@
 Default workload (the configuration the headline metric is quoted on): SISI
 intersect-path on one pair of sorted unique uint32 sets, |A| = |B| = 2^28,
 drawn uniformly from [0, 2^32), count plus materialised result list.  A step
-is one full intersection (merge-path kernel + placement kernel) of the
-HBM-resident pair; the timed region covers exactly K steps between two stream
+is one full intersection (one merge-path launch that counts, stages and places
+the matches) of the HBM-resident pair; the timed region covers exactly K steps between two stream
 events.  value = algorithmic bytes per step / seconds per step, with
 bytes = 4(|A|+|B|) + 4|A∩B| + 4.
 
@@ -125,7 +125,6 @@ def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
     ctx.kernel_timing(True)
     ms, wall = timed(ctx, lambda: ds.pair_materialize_async(0, 1, out, cnt), args.steps, args.warmup)
     k_ms = kavg(ctx, "ip_sisi_materialize")
-    c_ms = kavg(ctx, "ip_sisi_compact")
     value = nbytes / (ms / 1e3) / 1e9
     kernel_bytes = 4 * (na + nb) + 4 * exp.size
     achieved = kernel_bytes / (k_ms / 1e3) / 1e9
@@ -138,7 +137,7 @@ def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
         if kname:
             techs[name]["kernel_ms"] = round(kavg(ctx, kname) or 0, 4)
     log(f"[bench] materialise {ms:.4f} ms/step ({value:.0f} GB/s); merge kernel {k_ms:.4f} ms "
-        f"({achieved:.0f} GB/s), compact {c_ms:.4f} ms; counts: {techs}")
+        f"({achieved:.0f} GB/s); counts: {techs}")
     cpu = None if args.no_cpu else cpu_sisi(coll, nbytes, args.cpu_seconds, label)
     rec = base_record(args, value, "GB/s", ms,
                       {"workload": f"SISI intersect-path count+materialise, |A|=2^{log2a} |B|=2^{log2b} sorted "
@@ -147,8 +146,7 @@ def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
     rec["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBPS, 4),
                        "traffic": read_traffic(f"k_sisi_stream_materialise_2^{log2a}x2^{log2b}_pmc.md"),
-                       "kernel": SISI_KERNEL, "kernel_ms": round(k_ms, 4),
-                       "compact_kernel_ms": round(c_ms, 4)}
+                       "kernel": SISI_KERNEL, "kernel_ms": round(k_ms, 4)}
     rec["cpu_baseline"] = cpu
     rec["exact"] = exact
     rec["count_only"] = techs

@@ -31,7 +31,11 @@
 //    last workgroup to finish folds the per-segment partials (no atomics on
 //    the result, no memset before a launch).
 //  * Materialisation writes each segment's matches, in order, to a staging
-//    slot; a second kernel places every slot at its prefix-sum offset.
+//    slot with buffer stores that the hardware discards for non-matches, then
+//    places the slot at its prefix offset in the same launch (decoupled
+//    look-back over per-segment counts).
+#include <algorithm>
+
 #include "si_device.h"
 #include "si_internal.h"
 #include "si_merge.h"
@@ -41,67 +45,40 @@ namespace {
 using namespace si_dev;
 
 // ---------------------------------------------------------------- SISI ----
+// state[s] of a materialising launch: published flag (bit 63) | inclusive
+// flag (bit 62) | count (bits 0-32; a pair's intersection is below 2^33).
+// 0 means "not published"; the last workgroup clears the array, so launches
+// need no memset (and replay correctly from a captured graph).
+constexpr uint64_t kCountMask = (1ull << 33) - 1;
+constexpr uint64_t kPublished = 1ull << 63;
+constexpr uint64_t kInclusive = 1ull << 62;
+
 // One workgroup per merge-path segment of the pair (A, B).
-//   partials[s]     per-segment count (MAT) / partial sum (count mode)
-//   done            per-launch ticket counter, left at zero
-//   out             final uint64 count (count mode)
-template <int NT, int VT, bool MAT>
-__global__ __launch_bounds__(NT) void k_sisi_merge(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
-                                                   uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
-                                                   uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
-                                                   uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
-                                                   uint32_t* __restrict__ stage, uint64_t stage_cap) {
-    __shared__ WinSmem<NT, VT> sm;
-    const uint32_t s = blockIdx.x;
-    const uint64_t n = nA + nB;
-    const uint64_t dBeg = uint64_t(s) * seg_len;
-    const uint64_t dEnd = (dBeg + seg_len < n) ? dBeg + seg_len : n;
-
-    if (threadIdx.x < WAVE) {
-        const uint64_t dd = (lane_id() < 32) ? dBeg : dEnd;
-        const uint64_t c = corank_halfwave(base + oA, nA, base + oB, nB, dd);
-        if (lane_id() == 0) sm.seg[0] = c;
-        if (lane_id() == 32) sm.seg[1] = c;
-    }
-    __syncthreads();
-    const uint64_t ia = sm.seg[0], iaE = sm.seg[1];
-    const uint64_t ib = dBeg - ia, ibE = dEnd - iaE;
-
-    uint64_t c = merge_segment_win<NT, VT, MAT>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm,
-                                                MAT ? stage + uint64_t(s) * stage_cap : nullptr);
-    const uint64_t total = block_sum_u64<NT>(c, sm.red);
-    if (threadIdx.x == 0) partials[s] = total;
-    if (MAT) return;
-
-    // last workgroup folds the partials
-    __shared__ uint32_t isLast;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t ticket = atomicAdd(done, 1u);
-        isLast = (ticket == nseg - 1);
-    }
-    __syncthreads();
-    if (!isLast) return;
-    __threadfence();
-    uint64_t acc = 0;
-    for (uint32_t i = threadIdx.x; i < nseg; i += NT) acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    acc = block_sum_u64<NT>(acc, sm.red);
-    if (threadIdx.x == 0) {
-        if (out) *out = acc;
-        if (out32) *out32 = uint32_t(acc);
-        *done = 0;
-    }
-}
-
-// Same contract as k_sisi_merge, with half windows (merge_segment_stream).
+// Count mode: partials[s] holds segment s's count; the last workgroup to
+// finish (ticket done[0]) folds them into out / out32 and resets done[0].
+// Materialise mode: segments are numbered by a second ticket (done[1]), so
+// the predecessors of every segment are already running.  A segment stages
+// its matches in its slot of `stage`, publishes its count to state[s], adds
+// up its predecessors' counts (decoupled look-back, 64 predecessors per round:
+// an inclusive prefix ends the walk) and copies the slot to list[prefix ...).  The last segment writes
+// the total to out; the last workgroup to finish clears state and resets
+// done[0] and done[1].
 template <int NT, int VT, int WN, bool MAT, bool SEL>
 __global__ __launch_bounds__(NT) void k_sisi_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
                                                     uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
                                                     uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
                                                     uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
-                                                    uint32_t* __restrict__ stage, uint64_t stage_cap) {
+                                                    uint32_t* __restrict__ stage, uint64_t stage_cap,
+                                                    uint32_t* __restrict__ list, uint64_t* __restrict__ state) {
     __shared__ StreamSmem<NT, VT, WN> sm;
-    const uint32_t s = blockIdx.x;
+    __shared__ uint32_t shTicket;
+    __shared__ uint64_t shPrefix;
+    uint32_t s = blockIdx.x;
+    if (MAT) {
+        if (threadIdx.x == 0) shTicket = atomicAdd(&done[1], 1u);
+        __syncthreads();
+        s = shTicket;
+    }
     const uint64_t n = nA + nB;
     const uint64_t dBeg = uint64_t(s) * seg_len;
     const uint64_t dEnd = (dBeg + seg_len < n) ? dBeg + seg_len : n;
@@ -116,12 +93,66 @@ __global__ __launch_bounds__(NT) void k_sisi_stream(const uint32_t* __restrict__
     const uint64_t ia = sm.seg[0], iaE = sm.seg[1];
     const uint64_t ib = dBeg - ia, ibE = dEnd - iaE;
 
-    uint64_t c = merge_segment_stream<NT, VT, WN, MAT, SEL>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm,
-                                                       MAT ? stage + uint64_t(s) * stage_cap : nullptr);
+    uint32_t* slot = MAT ? stage + uint64_t(s) * stage_cap : nullptr;
+    const uint64_t c = merge_segment_stream<NT, VT, WN, MAT, SEL>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm, slot,
+                                                                  stage_cap);
     const uint64_t total = block_sum_u64<NT>(c, sm.red);
-    if (threadIdx.x == 0) partials[s] = total;
-    if (MAT) return;
 
+    if (MAT) {
+        if (threadIdx.x < WAVE) {
+            // wave 0 looks back 64 predecessors per round: lane l reads
+            // state[j - l]; the closest inclusive state ends the walk once
+            // every state up to it is published (before segment 0 there is
+            // an implicit inclusive 0)
+            const uint32_t lane = lane_id();
+            // the counts are the whole payload: relaxed atomics suffice, and
+            // polling without acquire keeps the CU's L1 intact for the
+            // workgroups still merging beside this one
+            if (lane == 0) __hip_atomic_store(&state[s], kPublished | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t pre = 0;
+            int64_t j = int64_t(s) - 1;
+            while (j >= 0) {
+                const int64_t idx = j - int64_t(lane);
+                const uint64_t v = idx >= 0 ? __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                            : (kPublished | kInclusive);
+                const uint64_t incl = __ballot((v & kInclusive) != 0);
+                const uint64_t pub = __ballot((v & kPublished) != 0);
+                // lanes [0, last] must be published: up to the closest inclusive, else all 64
+                const uint32_t last = incl ? uint32_t(__builtin_ctzll(incl)) : WAVE - 1;
+                const uint64_t need = last == WAVE - 1 ? ~0ull : ((2ull << last) - 1);
+                if ((pub & need) != need) {
+                    __builtin_amdgcn_s_sleep(16);
+                    continue;
+                }
+                pre += wave_sum_u64(lane <= last ? (v & kCountMask) : 0ull);
+                if (incl) break;
+                j -= WAVE;
+            }
+            if (lane == 0) {
+                __hip_atomic_store(&state[s], kPublished | kInclusive | (pre + total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                shPrefix = pre;
+                if (s == nseg - 1 && out) *out = pre + total;
+            }
+        }
+        __syncthreads();
+        const uint64_t pre = shPrefix;
+        for (uint64_t i = threadIdx.x; i < total; i += NT) list[pre + i] = slot[i];
+        if (threadIdx.x == 0) shTicket = atomicAdd(&done[0], 1u);
+        __syncthreads();
+        if (shTicket == nseg - 1) {  // every segment has finished its look-back
+            for (uint32_t i = threadIdx.x; i < nseg; i += NT)
+                __hip_atomic_store(&state[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (threadIdx.x == 0) {
+                __hip_atomic_store(&done[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&done[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        return;
+    }
+
+    // count mode: the last workgroup folds the partials
+    if (threadIdx.x == 0) partials[s] = total;
     __shared__ uint32_t isLast;
     if (threadIdx.x == 0) {
         __threadfence();
@@ -139,23 +170,6 @@ __global__ __launch_bounds__(NT) void k_sisi_stream(const uint32_t* __restrict__
         if (out32) *out32 = uint32_t(acc);
         *done = 0;
     }
-}
-
-// Places each segment's staged matches at its prefix offset.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ counts, uint32_t nseg,
-                                               const uint32_t* __restrict__ stage, uint64_t stage_cap,
-                                               uint32_t* __restrict__ out, uint64_t* __restrict__ total) {
-    __shared__ uint64_t red[NT / WAVE];
-    const uint32_t s = blockIdx.x;
-    uint64_t acc = 0;
-    for (uint32_t i = threadIdx.x; i < s; i += NT) acc += counts[i];
-    const uint64_t before = block_sum_u64<NT>(acc, red);
-    const uint64_t cnt = counts[s];
-    const uint32_t* src = stage + uint64_t(s) * stage_cap;
-    uint32_t* dst = out + before;
-    for (uint64_t i = threadIdx.x; i < cnt; i += NT) dst[i] = src[i];
-    if (s == nseg - 1 && threadIdx.x == 0) *total = before + cnt;
 }
 
 // ---------------------------------------------------------------- MISI ----
@@ -196,16 +210,15 @@ constexpr int RUN_VT = 8;
 
 namespace si_impl {
 
-// Tile shapes of the SISI merge kernel (threads x elements per thread) and
-// workgroups per CU; shape 0 is the default, the others are selectable
+// Tile shapes of the SISI merge kernel (threads x elements per thread, window
+// W) and workgroups per CU; shape 0 is the default, the others are selectable
 // through si_params.reserved[0] (count) or SI_SISI_SHAPE (materialisation)
-// for tuning sweeps.  Shapes 0-5 are k_sisi_stream (half windows; 1 reads
-// one head per step), 6-7 k_sisi_merge (T-element windows).
+// for tuning sweeps; shape 1 reads one head per merge step.
 struct SisiShape {
     int nt, vt, per_cu;
 };
-static const SisiShape kShapes[] = {{256, 31, 4}, {256, 31, 4}, {128, 31, 8}, {256, 15, 8},
-                                    {128, 15, 16}, {512, 15, 4}, {256, 15, 4}, {512, 15, 4}};
+static const SisiShape kShapes[] = {{256, 31, 4}, {256, 31, 4}, {128, 31, 8}, {256, 15, 8}, {128, 15, 16},
+                                    {512, 15, 4}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, const SisiShape& sh, uint32_t& nseg,
@@ -222,38 +235,38 @@ static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, const
     if (nseg == 0) nseg = 1;
 }
 
-template <int NT, int VT, bool MAT>
-static void launch_sisi(si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA, uint64_t oB,
-                        uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done, uint64_t* out,
-                        uint32_t* out32, uint32_t* stage, uint64_t cap) {
-    hipLaunchKernelGGL((k_sisi_merge<NT, VT, MAT>), dim3(nseg), dim3(NT), 0, ctx->stream, base, oA, nA, oB, nB,
-                       seg_len, nseg, partials, done, out, out32, stage, cap);
-}
+struct SisiLaunch {
+    si_context* ctx;
+    uint32_t nseg;
+    const uint32_t* base;
+    uint64_t oA, nA, oB, nB, seg_len;
+    uint64_t* partials;
+    uint32_t* done;
+    uint64_t* out;
+    uint32_t* out32;
+    uint32_t* stage;
+    uint64_t cap;
+    uint32_t* list;
+    uint64_t* state;
+};
 
 template <int NT, int VT, int WN, bool SEL, bool MAT>
-static void launch_stream(si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA,
-                          uint64_t oB, uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done,
-                          uint64_t* out, uint32_t* out32, uint32_t* stage, uint64_t cap) {
-    hipLaunchKernelGGL((k_sisi_stream<NT, VT, WN, MAT, SEL>), dim3(nseg), dim3(NT), 0, ctx->stream, base, oA, nA, oB, nB,
-                       seg_len, nseg, partials, done, out, out32, stage, cap);
+static void launch_stream(const SisiLaunch& L) {
+    hipLaunchKernelGGL((k_sisi_stream<NT, VT, WN, MAT, SEL>), dim3(L.nseg), dim3(NT), 0, L.ctx->stream, L.base, L.oA,
+                       L.nA, L.oB, L.nB, L.seg_len, L.nseg, L.partials, L.done, L.out, L.out32, L.stage, L.cap,
+                       L.list, L.state);
 }
 
 template <bool MAT>
-static void launch_shape(int shape, si_context* ctx, uint32_t nseg, const uint32_t* base, uint64_t oA, uint64_t nA,
-                         uint64_t oB, uint64_t nB, uint64_t seg_len, uint64_t* partials, uint32_t* done, uint64_t* out,
-                         uint32_t* out32, uint32_t* stage, uint64_t cap) {
-#define SI_SISI_ARGS ctx, nseg, base, oA, nA, oB, nB, seg_len, partials, done, out, out32, stage, cap
+static void launch_shape(int shape, const SisiLaunch& L) {
     switch (shape) {
-        case 1: launch_stream<256, 31, 4096, true, MAT>(SI_SISI_ARGS); break;
-        case 2: launch_stream<128, 31, 2048, false, MAT>(SI_SISI_ARGS); break;
-        case 3: launch_stream<256, 15, 2048, false, MAT>(SI_SISI_ARGS); break;
-        case 4: launch_stream<128, 15, 1024, false, MAT>(SI_SISI_ARGS); break;
-        case 5: launch_stream<512, 15, 4096, false, MAT>(SI_SISI_ARGS); break;
-        case 6: launch_sisi<256, 15, MAT>(SI_SISI_ARGS); break;
-        case 7: launch_sisi<512, 15, MAT>(SI_SISI_ARGS); break;
-        default: launch_stream<256, 31, 4096, false, MAT>(SI_SISI_ARGS); break;
+        case 1: launch_stream<256, 31, 4096, true, MAT>(L); break;
+        case 2: launch_stream<128, 31, 2048, false, MAT>(L); break;
+        case 3: launch_stream<256, 15, 2048, false, MAT>(L); break;
+        case 4: launch_stream<128, 15, 1024, false, MAT>(L); break;
+        case 5: launch_stream<512, 15, 4096, false, MAT>(L); break;
+        default: launch_stream<256, 31, 4096, false, MAT>(L); break;
     }
-#undef SI_SISI_ARGS
 }
 
 static int shape_of(const si_params* p) {
@@ -271,12 +284,12 @@ int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, ui
     sisi_geometry(ctx, nA + nB, p, kShapes[shape], nseg, seg_len);
     void* ws = nullptr;
     SI_TRY(small_workspace(ctx, 256 + uint64_t(nseg) * 8, &ws));
-    uint32_t* done = static_cast<uint32_t*>(ws);
-    uint64_t* partials = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256);
     if (nA == 0 || nB == 0) return zero_outputs(ctx, d_out, d_out32);
+    SisiLaunch L{ctx, nseg, ds->d_elems, oA, nA, oB, nB, seg_len,
+                 reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256), static_cast<uint32_t*>(ws), d_out,
+                 d_out32, nullptr, 0, nullptr, nullptr};
     KernelScope ks(ctx, "ip_sisi_count");
-    launch_shape<false>(shape, ctx, nseg, ds->d_elems, oA, nA, oB, nB, seg_len, partials, done, d_out, d_out32,
-                        nullptr, 0);
+    launch_shape<false>(shape, L);
     SI_HIP(hipGetLastError());
     return SI_OK;
 }
@@ -293,25 +306,21 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
     uint32_t nseg;
     uint64_t seg_len;
     sisi_geometry(ctx, nA + nB, nullptr, kShapes[shape], nseg, seg_len);
+    // a segment of seg_len merged elements holds at most seg_len / 2 matches
     const uint64_t cap = ((seg_len / 2 + 2) + 63) & ~uint64_t(63);
     void* ws = nullptr;
-    SI_TRY(small_workspace(ctx, 256 + uint64_t(nseg) * 8, &ws));
-    uint32_t* done = static_cast<uint32_t*>(ws);
-    uint64_t* partials = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256);
+    SI_TRY(small_workspace(ctx, 256, &ws));
     SI_TRY(ctx->ws_stage.ensure(cap * nseg * sizeof(uint32_t)));
-    uint32_t* stage = ctx->ws_stage.as<uint32_t>();
-    {
-        KernelScope ks(ctx, "ip_sisi_materialize");
-        launch_shape<true>(shape, ctx, nseg, ds->d_elems, oA, nA, oB, nB, seg_len, partials, done, d_count, nullptr,
-                           stage, cap);
-        SI_HIP(hipGetLastError());
+    // look-back states: zeroed once when (re)allocated, left zeroed by every launch
+    if (ctx->ws_lookback.bytes < uint64_t(nseg) * 8) {
+        SI_TRY(ctx->ws_lookback.ensure(std::max<uint64_t>(uint64_t(nseg) * 8, 1 << 14)));
+        SI_HIP(hipMemsetAsync(ctx->ws_lookback.ptr, 0, ctx->ws_lookback.bytes, ctx->stream));
     }
-    {
-        KernelScope ks(ctx, "ip_sisi_compact");
-        hipLaunchKernelGGL((k_compact<256>), dim3(nseg), dim3(256), 0, ctx->stream, partials, nseg, stage, cap, d_out,
-                           d_count);
-        SI_HIP(hipGetLastError());
-    }
+    SisiLaunch L{ctx, nseg, ds->d_elems, oA, nA, oB, nB, seg_len, nullptr, static_cast<uint32_t*>(ws), d_count,
+                 nullptr, ctx->ws_stage.as<uint32_t>(), cap, d_out, ctx->ws_lookback.as<uint64_t>()};
+    KernelScope ks(ctx, "ip_sisi_materialize");
+    launch_shape<true>(shape, L);
+    SI_HIP(hipGetLastError());
     return SI_OK;
 }
 

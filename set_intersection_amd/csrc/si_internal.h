@@ -69,6 +69,7 @@ struct si_context {
     // scratch workspaces (reused across calls on this context)
     si_impl::DevMem ws_small;   // counters, per-segment counts, reduction partials
     si_impl::DevMem ws_stage;   // materialisation staging
+    si_impl::DevMem ws_lookback;  // SISI materialisation look-back states (zero between launches)
     si_impl::DevMem ws_aux;     // technique-specific (bitmaps, hash bins, pair lists)
     si_impl::DevMem ws_aux2;
     bool ws_small_zeroed = false;

@@ -296,263 +296,6 @@ __device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA
 }
 
 // --------------------------------------------------------------------------
-// Streaming merge for long segments (SISI), for any VT.  Each side keeps a
-// circular window indexed modulo W, the power of two >= T = NT*VT; the window
-// holds A[ia, ia + T) (B[ib, ib + T)) and after a tile consuming ca elements
-// of A the positions [ia + T, ia + T + ca) are fetched into the freed slots.
-// Lanes read the windows about VT/2 words apart per side: an even VT puts
-// whole groups of lanes on the same LDS bank, an odd VT spreads them over all
-// banks, which is why the tile need not be a power of two.
-constexpr uint32_t pow2_ceil(uint32_t v) {
-    uint32_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
-
-template <int NT, int VT>
-struct WinCfg {
-    static constexpr uint32_t T = uint32_t(NT) * VT;   // merged elements per tile
-    static constexpr uint32_t W = pow2_ceil(T);        // window modulus per side
-    static constexpr uint32_t M = W - 1;
-    static constexpr int CPT = int((T / 4 + 4 + NT - 1) / NT);  // 16-byte chunks per thread per refill
-};
-
-template <int NT, int VT>
-struct WinSmem {
-    uint32_t win[2 * WinCfg<NT, VT>::W];  // A window [0, W), B window [W, 2W)
-    uint32_t ends[NT];
-    uint32_t wave_tot[NT / WAVE];
-    uint64_t red[NT / WAVE];
-    uint64_t seg[4];
-};
-
-// Loads of one refill (at most T elements over both sides), held in registers
-// between issue and commit.  Every lane issues all CPT loads back to back;
-// chunk numbers past the needed ones are clamped to the last needed chunk (in
-// bounds, not committed).
-template <int NT, int VT>
-struct WinLoad {
-    using C = WinCfg<NT, VT>;
-    uint4 v[C::CPT];
-    uint32_t nCA, nCB, phA, phB, lenA, lenB, sA, sB;
-
-    __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t a0, uint64_t a1, uint64_t b0,
-                                          uint64_t b1) {
-        const uint4* b4 = reinterpret_cast<const uint4*>(base);
-        lenA = a1 > a0 ? uint32_t(a1 - a0) : 0u;
-        lenB = b1 > b0 ? uint32_t(b1 - b0) : 0u;
-        nCA = lenA ? uint32_t(((a1 + 3) >> 2) - (a0 >> 2)) : 0u;
-        nCB = lenB ? uint32_t(((b1 + 3) >> 2) - (b0 >> 2)) : 0u;
-        phA = uint32_t(a0 & 3);
-        phB = uint32_t(b0 & 3);
-        sA = uint32_t(a0 & ~3ull) & C::M;
-        sB = uint32_t(b0 & ~3ull) & C::M;
-        const uint4* pA = b4 + (a0 >> 2);
-        const uint4* pB = b4 + (b0 >> 2);
-        const uint32_t tot = nCA + nCB;
-        if (tot) {
-#pragma unroll
-            for (int j = 0; j < C::CPT; ++j) {
-                uint32_t c = threadIdx.x + uint32_t(j) * NT;
-                c = c < tot ? c : tot - 1;
-                const uint4* p = c < nCA ? pA + c : pB + (c - nCA);
-                v[j] = *p;
-            }
-        }
-    }
-
-    // chunk c of a range holds elements rel = 4c - ph + q, needed for
-    // 0 <= rel < len; an aligned chunk stays contiguous in the window (W % 4 == 0)
-    __device__ __forceinline__ static void put(uint32_t* w, const uint4& x, uint32_t c, uint32_t ph, uint32_t len,
-                                               uint32_t s0) {
-        const int rel = int(4 * c) - int(ph);
-        const uint32_t slot = (s0 + 4 * c) & C::M;
-        if (rel >= 0 && rel + 4 <= int(len)) {
-            *reinterpret_cast<uint4*>(&w[slot]) = x;
-        } else {
-            const uint32_t e[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (rel + q >= 0 && rel + q < int(len)) w[slot + q] = e[q];
-        }
-    }
-
-    __device__ __forceinline__ void commit(WinSmem<NT, VT>& sm) const {
-#pragma unroll
-        for (int j = 0; j < C::CPT; ++j) {
-            const uint32_t c = threadIdx.x + uint32_t(j) * NT;
-            if (c < nCA)
-                put(sm.win, v[j], c, phA, lenA, sA);
-            else if (c < nCA + nCB)
-                put(sm.win + C::W, v[j], c - nCA, phB, lenB, sB);
-        }
-    }
-};
-
-// Streams the merge path from (ia, ib) to (iaEnd, ibEnd) of sets A (global
-// element offset oA, size nA) and B (oB, nB).  Returns this thread's match
-// count; with MAT the workgroup's matches are written in order to stage[0 ..).
-template <int NT, int VT, bool MAT>
-__device__ uint64_t merge_segment_win(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
-                                      uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
-                                      WinSmem<NT, VT>& sm, uint32_t* __restrict__ stage) {
-    using C = WinCfg<NT, VT>;
-    constexpr uint32_t T = C::T;
-    constexpr uint32_t M = C::M;
-    const uint32_t tid = threadIdx.x;
-    uint64_t myCount = 0;
-    uint64_t written = 0;
-
-    uint64_t d = ia + ib;
-    const uint64_t dEnd = iaEnd + ibEnd;
-    if (d >= dEnd) return 0;
-
-    {
-        WinLoad<NT, VT> r;
-        const uint64_t a1 = ia + T < nA ? ia + T : nA;
-        r.issue(base, oA + ia, oA + a1, 0, 0);
-        r.commit(sm);
-        const uint64_t b1 = ib + T < nB ? ib + T : nB;
-        r.issue(base, 0, 0, oB + ib, oB + b1);
-        r.commit(sm);
-    }
-    __syncthreads();
-
-    const uint32_t* rA = sm.win;
-    const uint32_t* rB = sm.win + C::W;
-    while (d < dEnd) {
-        const uint32_t Tc = uint32_t((dEnd - d) < T ? (dEnd - d) : T);
-        const uint32_t nAw = uint32_t((nA - ia) < T ? (nA - ia) : T);
-        const uint32_t nBw = uint32_t((nB - ib) < T ? (nB - ib) : T);
-        const uint32_t sa = uint32_t(oA + ia) & M;  // window slot of A[ia]
-        const uint32_t sb = uint32_t(oB + ib) & M;
-        auto WA = [&](uint32_t i) { return rA[(sa + i) & M]; };
-        auto WB = [&](uint32_t i) { return rB[(sb + i) & M]; };
-        const bool full = (Tc == T) && (nAw == T) && (nBw == T);
-
-        // 1. split at the end of this thread's diagonal range
-        const uint32_t dd = ((tid + 1) * VT < Tc) ? (tid + 1) * VT : Tc;
-        {
-            uint32_t lo = dd > nBw ? dd - nBw : 0u;
-            uint32_t hi = dd < nAw ? dd : nAw;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (WA(mid) <= WB(dd - 1 - mid))
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            sm.ends[tid] = lo;
-        }
-        __syncthreads();
-        const uint32_t ds = (tid * VT < Tc) ? tid * VT : Tc;
-        const uint32_t x0 = tid ? sm.ends[tid - 1] : 0u;
-        const uint32_t x1 = sm.ends[tid];
-        const uint32_t y0 = ds - x0;
-        const uint32_t y1 = dd - x1;
-        const uint32_t ca = sm.ends[NT - 1];
-        const uint32_t cb = Tc - ca;
-
-        // 2. fetch the elements that replace the consumed ones (registers)
-        WinLoad<NT, VT> r;
-        {
-            const uint64_t a0 = ia + T < nA ? ia + T : nA;
-            const uint64_t a1 = ia + T + ca < nA ? ia + T + ca : nA;
-            const uint64_t b0 = ib + T < nB ? ib + T : nB;
-            const uint64_t b1 = ib + T + cb < nB ? ib + T + cb : nB;
-            r.issue(base, oA + a0, oA + a1, oB + b0, oB + b1);
-        }
-
-        // 3. merge this thread's VT steps; A is taken on ties and matched
-        //    against the current B head (which may lie past y1)
-        uint32_t cnt = 0;
-        uint32_t mval[MAT ? VT : 1];
-        uint32_t mbits = 0;
-        if (full) {
-            // Every thread has exactly VT steps and the heads A[x], B[y] are
-            // in the windows for x <= x1, y <= y1 (x1 == T or y1 == T only
-            // when the other side has no steps), so the comparison alone
-            // decides: take A iff A[x] <= B[y]; a tie is a match.
-            uint32_t pa = sa + x0, pb = C::W + sb + y0;  // unmasked positions
-            uint32_t av = rA[pa & M], bv = sm.win[C::W + ((pb - C::W) & M)];
-#pragma unroll
-            for (int k = 0; k < VT; ++k) {
-                const bool takeA = av <= bv;
-                const bool hit = av == bv;
-                cnt += hit;
-                if (MAT) {
-                    mval[k] = av;
-                    mbits |= uint32_t(hit) << k;
-                }
-                pa += takeA;
-                pb += !takeA;
-                const uint32_t slot = takeA ? (pa & M) : (C::W + ((pb - C::W) & M));
-                const uint32_t nv = sm.win[slot];
-                av = takeA ? nv : av;
-                bv = takeA ? bv : nv;
-            }
-        } else {
-            uint32_t x = x0, y = y0;
-            uint32_t av = x < x1 ? WA(x) : 0u;
-            uint32_t bv = y < nBw ? WB(y) : 0u;
-            const uint32_t steps = (x1 - x0) + (y1 - y0);
-#pragma unroll
-            for (int k = 0; k < VT; ++k) {
-                if (uint32_t(k) < steps) {
-                    const bool takeA = (x < x1) && (y >= y1 || av <= bv);
-                    const bool hit = takeA && (y < nBw) && (av == bv);
-                    cnt += hit;
-                    if (MAT) {
-                        mval[k] = av;
-                        mbits |= uint32_t(hit) << k;
-                    }
-                    if (takeA) {
-                        ++x;
-                        if (x < x1) av = WA(x);
-                    } else {
-                        ++y;
-                        if (y < nBw) bv = WB(y);
-                    }
-                }
-            }
-        }
-        myCount += cnt;
-
-        uint32_t excl = 0, wtot = 0;
-        if (MAT) {
-            const uint32_t incl = wave_inclusive_scan_u32(cnt);
-            excl = incl - cnt;
-            if (lane_id() == WAVE - 1) sm.wave_tot[tid / WAVE] = incl;
-        }
-        __syncthreads();  // all window reads of this tile are done
-
-        // 4. commit the refill; with MAT, stage this tile's matches
-        r.commit(sm);
-        if (MAT) {
-            const uint32_t w = tid / WAVE;
-            uint32_t before = 0;
-#pragma unroll
-            for (int i = 0; i < NT / WAVE; ++i) {
-                const uint32_t t = sm.wave_tot[i];
-                if (i < int(w)) before += t;
-                wtot += t;
-            }
-            uint32_t* dst = stage + written + before + excl;
-#pragma unroll
-            for (int k = 0; k < VT; ++k) {
-                if (mbits & (1u << k)) *dst++ = mval[k];
-            }
-            written += wtot;
-        }
-        ia += ca;
-        ib += cb;
-        d += Tc;
-        __syncthreads();
-    }
-    return myCount;
-}
-
-// --------------------------------------------------------------------------
 // Half-window streaming merge (SISI).  A tile of up to T = NT*VT merged
 // elements consumes about T/2 from each side, so each side keeps only a window
 // of W elements (a power of two a little above T/2): the window holds
@@ -682,10 +425,12 @@ __device__ __forceinline__ uint32_t split_search(const uint32_t* wa, uint32_t sa
 
 // SEL: one LDS read per merge step (the next element of the side just
 // consumed, selected by address) instead of re-reading both heads.
+// With MAT, stage_cap is the staging slot's capacity in elements.
 template <int NT, int VT, int WN, bool MAT, bool SEL = false>
 __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
                                          uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
-                                         StreamSmem<NT, VT, WN>& sm, uint32_t* __restrict__ stage) {
+                                         StreamSmem<NT, VT, WN>& sm, uint32_t* __restrict__ stage,
+                                         uint64_t stage_cap = 0) {
     using C = StreamCfg<NT, VT, WN>;
     constexpr uint32_t T = C::T, W = C::W, M = C::M;
     const uint32_t tid = threadIdx.x;

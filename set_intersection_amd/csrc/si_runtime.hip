@@ -205,6 +205,7 @@ int si_close(si_context* ctx) {
     for (auto e : ctx->free_events) (void)hipEventDestroy(e);
     ctx->ws_small.release();
     ctx->ws_stage.release();
+    ctx->ws_lookback.release();
     ctx->ws_aux.release();
     ctx->ws_aux2.release();
     (void)hipStreamDestroy(ctx->stream);
