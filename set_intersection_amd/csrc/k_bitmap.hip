@@ -33,12 +33,42 @@ namespace {
 using namespace si_dev;
 
 // ------------------------------------------------------- bitmap_dynamic --
-__global__ void k_bm_build(const uint32_t* __restrict__ S, uint64_t n, uint32_t* __restrict__ bm) {
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t x = S[i];
-        atomicOr(&bm[x >> 5], 1u << (x & 31));
+// Sets bit x of the 32-bit-word bitmap bm for the elements x of the sorted
+// unique list S[c*64, c*64+64) handled by this wave (chunk c).  Elements of
+// one word are consecutive, so lanes that share a word OR their bits together
+// (segmented OR over the wave with shuffles) and the segment's first lane
+// writes the word: a plain store inside the chunk, an atomic OR only for the
+// chunk's first and last word, which the neighbouring chunks may share.
+// bm must be zero beforehand.
+__device__ __forceinline__ void or_bits_chunk(const uint32_t* __restrict__ S, uint64_t n, uint64_t c,
+                                              uint32_t* __restrict__ bm) {
+    const uint32_t lane = lane_id();
+    const uint64_t i = c * WAVE + lane;
+    const bool valid = i < n;
+    const uint32_t x = valid ? S[i] : 0u;
+    const uint32_t w = valid ? (x >> 5) : 0xFFFFFFFFu;
+    uint32_t bits = valid ? (1u << (x & 31)) : 0u;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t vb = __shfl_down(bits, off);
+        const uint32_t vw = __shfl_down(w, off);
+        if (lane + off < WAVE && vw == w) bits |= vb;
     }
+    const uint32_t wprev = __shfl_up(w, 1);
+    const uint32_t w0 = __shfl(w, 0), wlast = __shfl(w, WAVE - 1);
+    if (valid && (lane == 0 || wprev != w)) {
+        if (w == w0 || w == wlast)
+            atomicOr(&bm[w], bits);
+        else
+            bm[w] = bits;
+    }
+}
+
+__global__ void k_bm_build(const uint32_t* __restrict__ S, uint64_t n, uint32_t* __restrict__ bm) {
+    const uint64_t chunks = (n + WAVE - 1) / WAVE;
+    const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / WAVE);
+    for (uint64_t c = uint64_t(blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < chunks; c += waves)
+        or_bits_chunk(S, n, c, bm);
 }
 
 template <int NT>
@@ -98,15 +128,17 @@ __global__ __launch_bounds__(NT) void k_bm_run(const uint32_t* __restrict__ base
 
 // --------------------------------------------------------- bitmap_naive --
 // Bitmaps of sets [s0, s0 + ns): row (s - s0) has W64 words of 64 bits.
+// Bit rows of sets [s0, s0 + ns): row s is w64 64-bit words (2 * w64 32-bit
+// words, same bit order) starting at rows + s * w64.  rows must be zero.
 __global__ void k_bm_build_rows(const uint32_t* __restrict__ base, const uint64_t* __restrict__ offsets, uint32_t s0,
                                 uint32_t ns, uint64_t w64, unsigned long long* __restrict__ rows) {
+    const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / WAVE);
     for (uint32_t s = blockIdx.y; s < ns; s += gridDim.y) {
         const uint64_t o = offsets[s0 + s], n = offsets[s0 + s + 1] - o;
-        unsigned long long* row = rows + uint64_t(s) * w64;
-        for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
-            const uint32_t x = base[o + i];
-            atomicOr(&row[x >> 6], 1ull << (x & 63));
-        }
+        uint32_t* row = reinterpret_cast<uint32_t*>(rows + uint64_t(s) * w64);
+        const uint64_t chunks = (n + WAVE - 1) / WAVE;
+        for (uint64_t c = uint64_t(blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < chunks; c += waves)
+            or_bits_chunk(base + o, n, c, row);
     }
 }
 
@@ -252,9 +284,9 @@ int bitmap_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_
 }
 
 // Bit rows of sets [s0, s0+ns) into `rows` (ns x w64 words), zeroed first.
-int build_bit_rows(si_dataset* ds, uint32_t s0, uint32_t ns, uint64_t w64, unsigned long long* rows) {
+int build_bit_rows(si_dataset* ds, uint32_t s0, uint32_t ns, uint64_t w64, unsigned long long* rows, bool zero) {
     si_context* ctx = ds->ctx;
-    SI_HIP(hipMemsetAsync(rows, 0, uint64_t(ns) * w64 * 8, ctx->stream));
+    if (zero) SI_HIP(hipMemsetAsync(rows, 0, uint64_t(ns) * w64 * 8, ctx->stream));
     if (ns == 0) return SI_OK;
     KernelScope ks(ctx, "bitmap_build_rows");
     dim3 grid(8, std::min<uint32_t>(ns, 65535));

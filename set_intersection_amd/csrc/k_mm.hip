@@ -8,20 +8,29 @@
 // counts stay below 2^24.
 //
 // MI355X design:
-//  * Incidence matrices are built on the device from the sorted lists
-//    (one byte per bit, rows padded to 128, K padded to 64); nothing crosses
-//    PCIe but the counts.
-//  * C = M_A · M_Bᵀ with v_mfma_i32_16x16x64_i8: both operands are rows of
-//    row-major incidence matrices (K contiguous), so A and B fragments are the
-//    same 16-byte row slices.  int32 accumulation is exact below 2^31.
-//  * A 256-thread workgroup computes a 128 x 128 tile (2 x 2 waves of 64 x 64,
-//    4 x 4 MFMA blocks each); 64-byte K steps are staged through LDS with the
-//    next step's global loads issued before the current step's MFMAs.
+//  * Operands stay bit-packed: one 64-bit word per 64 universe values per set
+//    (8 KiB per set at 2^16), built on the device from the sorted lists; the
+//    bytes the matrix cores consume are made in registers right before each
+//    MFMA, so the operand traffic is an eighth of a byte matrix's.
+//  * C = M_A · M_Bᵀ with v_mfma_i32_16x16x64_i8: a lane's 16 operand bytes
+//    are one row's 16 consecutive universe values, i.e. one 16-bit slice of
+//    that row's 64-bit word, expanded nibble by nibble
+//    ((n * 0x204081) & 0x01010101 puts bit i of n in byte i).  int32
+//    accumulation is exact below 2^31.
+//  * A two-wave workgroup computes a 128 x 128 tile (each wave 128 x 64,
+//    8 x 4 MFMA blocks: an expanded fragment feeds 4 or 8 MFMAs).  Chunks of
+//    8 words (512 universe values) of the 256 operand rows are staged through
+//    double-buffered LDS (odd dword row stride: the 16 rows a fragment reads
+//    hit distinct banks); the next chunk's global loads are issued before the
+//    current chunk's MFMAs.
+//  * Whole tiles fill complete rounds of the resident workgroup slots; the
+//    tiles of a last partial round are split along K into up to 16 parts
+//    added atomically into zeroed C, so no round runs mostly empty.
 //  * For a self-join tile (A == B) only tiles on or above the diagonal are
 //    computed and each is stored twice (C and Cᵀ), halving the MFMA work.
-//  * The FP32 comparison variant uses v_mfma_f32_16x16x32_fp8_fp8 on the same
-//    matrices built with the fp8 (e4m3) encoding of 1.0 (0x38): exact only
-//    while every count is below 2^24.
+//  * The FP32-accumulate comparison variant expands the same bits to the fp8
+//    (e4m3) encoding of 1.0 (0x38) and uses v_mfma_f32_16x16x32_fp8_fp8:
+//    exact only while every count is below 2^24.
 #include <type_traits>
 
 #include "si_device.h"
@@ -35,117 +44,184 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int TILE = 128;
-constexpr int KS = 64;          // bytes of K per step
-constexpr int LDS_STRIDE = 80;  // padded row stride (bytes) in LDS
+constexpr int KC = 8;              // 64-bit words per staged chunk
+constexpr int RS = 2 * KC + 1;     // LDS row stride in dwords (odd)
 
-// rows [s0, s0+ns) of the collection -> mat[(s - s0) * Kp + x] = one
-__global__ void k_mm_build(const uint32_t* __restrict__ base, const uint64_t* __restrict__ offsets, uint32_t s0,
-                           uint32_t ns, uint64_t Kp, uint8_t one, uint8_t* __restrict__ mat) {
-    for (uint32_t s = blockIdx.y; s < ns; s += gridDim.y) {
-        const uint64_t o = offsets[s0 + s], n = offsets[s0 + s + 1] - o;
-        uint8_t* row = mat + uint64_t(s) * Kp;
-        for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
-            const uint32_t x = base[o + i];
-            if (x < Kp) row[x] = one;
-        }
+// 16 bits -> 16 bytes of 0/1 (int8) or 0/0x38 (fp8 1.0), byte i = bit i
+template <bool F8>
+__device__ __forceinline__ v4i expand16(uint32_t s16) {
+    v4i r;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        uint32_t v = __umul24(__builtin_amdgcn_ubfe(s16, 4 * d, 4), 0x204081u) & 0x01010101u;
+        if (F8) v *= 0x38u;
+        r[d] = int(v);
+    }
+    return r;
+}
+
+// Work unit u of a GEMM launch: units [0, nfull) are whole tiles; each of
+// the ntail tiles after them is split into nsplit K ranges (units nfull + ...)
+// whose partial counts are added atomically into zeroed C.
+struct MmGrid {
+    uint32_t nfull, ntail, nsplit, tilesN;
+    bool self;
+};
+
+__device__ __forceinline__ void mm_tile_of(const MmGrid& g, uint32_t t, uint32_t& tm, uint32_t& tn) {
+    if (g.self) {
+        tri_decode(t, g.tilesN + 1, tm, tn);
+        tn -= 1;
+    } else {
+        tm = t / g.tilesN;
+        tn = t % g.tilesN;
     }
 }
 
+// C(i, j) = |row i ∩ row j| for a 128 x 128 tile of bit rows RA (ni rows) x
+// RB (nj rows), w64 words per row (a multiple of KC; rows padded to a
+// multiple of 128 with zeros).  Two waves: wave w computes all 128 rows x
+// columns [64w, 64w + 64) (8 x 4 MFMA blocks), so an expanded fragment feeds
+// 4 (A) or 8 (B) MFMAs.  self: RA == RB, tiles on or above the diagonal only,
+// each stored to C and Cᵀ.
 template <bool F32>
-__global__ __launch_bounds__(256) void k_mm_gemm(const uint8_t* __restrict__ MA, uint32_t ni,
-                                                 const uint8_t* __restrict__ MB, uint32_t nj, uint64_t Kp, bool self,
-                                                 uint32_t tilesN, uint32_t* __restrict__ C, uint64_t ld) {
-    __shared__ __attribute__((aligned(16))) uint8_t As[TILE * LDS_STRIDE];
-    __shared__ __attribute__((aligned(16))) uint8_t Bs[TILE * LDS_STRIDE];
-    uint32_t tm, tn;
-    if (self) {
-        tri_decode(blockIdx.x, tilesN + 1, tm, tn);
-        tn -= 1;
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mm_bits(const uint64_t* __restrict__ RA, uint32_t ni,
+                                                 const uint64_t* __restrict__ RB, uint32_t nj, uint64_t w64,
+                                                 MmGrid g, uint32_t* __restrict__ C, uint64_t ld) {
+    __shared__ uint32_t sA[2][TILE * RS];
+    __shared__ uint32_t sB[2][TILE * RS];
+    const uint32_t u = blockIdx.x;
+    uint32_t tm, tn, c0 = 0, c1 = uint32_t(w64 / KC);
+    bool split = false;
+    if (u < g.nfull) {
+        mm_tile_of(g, u, tm, tn);
     } else {
-        tm = blockIdx.x / tilesN;
-        tn = blockIdx.x % tilesN;
+        const uint32_t v = u - g.nfull, part = v % g.nsplit;
+        mm_tile_of(g, g.nfull + v / g.nsplit, tm, tn);
+        const uint32_t nch = c1;
+        c0 = uint32_t(uint64_t(nch) * part / g.nsplit);
+        c1 = uint32_t(uint64_t(nch) * (part + 1) / g.nsplit);
+        split = true;
     }
-    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t wm = w >> 1, wn = w & 1;
-    const uint8_t* gA = MA + uint64_t(tm) * TILE * Kp;
-    const uint8_t* gB = MB + uint64_t(tn) * TILE * Kp;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wn = tid >> 6;
+    const uint64_t* gA = RA + uint64_t(tm) * TILE * w64;
+    const uint64_t* gB = RB + uint64_t(tn) * TILE * w64;
 
-    // loader: 2 x 16 B of A and of B per thread per K step
-    const uint32_t q0 = tid, q1 = tid + 256u;
-    const uint64_t offA0 = uint64_t(q0 >> 2) * Kp + (q0 & 3) * 16, offA1 = uint64_t(q1 >> 2) * Kp + (q1 & 3) * 16;
-    const uint32_t lds0 = (q0 >> 2) * LDS_STRIDE + (q0 & 3) * 16, lds1 = (q1 >> 2) * LDS_STRIDE + (q1 & 3) * 16;
-    uint4 ra0 = *reinterpret_cast<const uint4*>(gA + offA0), ra1 = *reinterpret_cast<const uint4*>(gA + offA1);
-    uint4 rb0 = *reinterpret_cast<const uint4*>(gB + offA0), rb1 = *reinterpret_cast<const uint4*>(gB + offA1);
+    // loader: a chunk is 128 rows x 8 words per side = 512 x 16 B; thread t
+    // moves pieces t + 128 p (row = piece / 4, words 2 * (piece % 4) + {0,1})
+    uint64_t goff[4];
+    uint32_t loff[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t piece = tid + 128u * p, row = piece >> 2, c2 = (piece & 3) * 2;
+        goff[p] = uint64_t(row) * w64 + c2;
+        loff[p] = row * RS + 2 * c2;
+    }
+    uint4 ra[4], rb[4];
+    auto load = [&](uint64_t k) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            ra[p] = *reinterpret_cast<const uint4*>(gA + goff[p] + k);
+            rb[p] = *reinterpret_cast<const uint4*>(gB + goff[p] + k);
+        }
+    };
+    auto store = [&](uint32_t buf) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            uint32_t* da = &sA[buf][loff[p]];
+            uint32_t* db = &sB[buf][loff[p]];
+            da[0] = ra[p].x; da[1] = ra[p].y; da[2] = ra[p].z; da[3] = ra[p].w;
+            db[0] = rb[p].x; db[1] = rb[p].y; db[2] = rb[p].z; db[3] = rb[p].w;
+        }
+    };
 
     using Acc = typename std::conditional<F32, v4f, v4i>::type;
-    Acc acc[4][4];
+    Acc acc[8][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 8; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
-    const uint32_t r16 = lane & 15, kq = lane >> 4;
-    for (uint64_t k0 = 0; k0 < Kp; k0 += KS) {
-        *reinterpret_cast<uint4*>(&As[lds0]) = ra0;
-        *reinterpret_cast<uint4*>(&As[lds1]) = ra1;
-        *reinterpret_cast<uint4*>(&Bs[lds0]) = rb0;
-        *reinterpret_cast<uint4*>(&Bs[lds1]) = rb1;
-        __syncthreads();
-        if (k0 + KS < Kp) {
-            const uint64_t k1 = k0 + KS;
-            ra0 = *reinterpret_cast<const uint4*>(gA + offA0 + k1);
-            ra1 = *reinterpret_cast<const uint4*>(gA + offA1 + k1);
-            rb0 = *reinterpret_cast<const uint4*>(gB + offA0 + k1);
-            rb1 = *reinterpret_cast<const uint4*>(gB + offA1 + k1);
-        }
-        if constexpr (!F32) {
-            v4i a[4], b[4];
+    // a lane's operand rows: 16 * i + (lane & 15); its 16-bit slice of a word
+    // is number (lane >> 4) (halfword 4 kk + q of the row's chunk)
+    const uint32_t r16 = lane & 15, q = lane >> 4;
+    const uint32_t rowA = r16 * RS, rowB = (wn * 64 + r16) * RS;
+    load(uint64_t(c0) * KC);
+    store(0);
+    __syncthreads();
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t buf = (c - c0) & 1;
+        if (c + 1 < c1) load(uint64_t(c + 1) * KC);
+        const uint16_t* hA = reinterpret_cast<const uint16_t*>(sA[buf]);
+        const uint16_t* hB = reinterpret_cast<const uint16_t*>(sB[buf]);
+        for (int kk = 0; kk < KC; ++kk) {
+            v4i fb[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                a[i] = *reinterpret_cast<const v4i*>(&As[(wm * 64 + i * 16 + r16) * LDS_STRIDE + kq * 16]);
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t sb = hB[2 * (rowB + j * 16 * RS) + 4 * kk + q];
+                fb[j] = expand16<F32>(sb);
+            }
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                b[j] = *reinterpret_cast<const v4i*>(&Bs[(wn * 64 + j * 16 + r16) * LDS_STRIDE + kq * 16]);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
-        } else {
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                long a[4], b[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    a[i] = *reinterpret_cast<const long*>(&As[(wm * 64 + i * 16 + r16) * LDS_STRIDE + s * 32 + kq * 8]);
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    b[j] = *reinterpret_cast<const long*>(&Bs[(wn * 64 + j * 16 + r16) * LDS_STRIDE + s * 32 + kq * 8]);
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t sa = hA[2 * (rowA + i * 16 * RS) + 4 * kk + q];
+                const v4i fa = expand16<F32>(sa);
+                if constexpr (!F32) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], b[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[j], acc[i][j], 0, 0, 0);
+                } else {
+                    // the fp8 MFMA takes 8 bytes per lane: the fragment's two
+                    // halves are two K sub-steps (the same for A and B)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const long av = (long(uint32_t(fa[2 * h + 1])) << 32) | uint32_t(fa[2 * h]);
+                            const long bv = (long(uint32_t(fb[j][2 * h + 1])) << 32) | uint32_t(fb[j][2 * h]);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, bv, acc[i][j], 0, 0, 0);
+                        }
+                }
             }
         }
+        if (c + 1 < c1) store(buf ^ 1);
         __syncthreads();
     }
 
     // lane holds column (lane & 15) and rows 4 * (lane >> 4) + v of each block
-    const bool mirror = self && tm != tn;
+    const bool mirror = g.self && tm != tn;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 8; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const uint32_t row = tm * TILE + wm * 64 + i * 16 + 4 * (lane >> 4) + v;
+                const uint32_t row = tm * TILE + i * 16 + 4 * (lane >> 4) + v;
                 const uint32_t col = tn * TILE + wn * 64 + j * 16 + (lane & 15);
                 const uint32_t val = uint32_t(acc[i][j][v]);
                 if (row < ni && col < nj) {
-                    C[uint64_t(row) * ld + col] = val;
-                    if (mirror) C[uint64_t(col) * ld + row] = val;
+                    if (split) {
+                        atomicAdd(&C[uint64_t(row) * ld + col], val);
+                        if (mirror) atomicAdd(&C[uint64_t(col) * ld + row], val);
+                    } else {
+                        C[uint64_t(row) * ld + col] = val;
+                        if (mirror) C[uint64_t(col) * ld + row] = val;
+                    }
                 }
             }
+}
+
+// Zeroes the C regions of the tail tiles [t0, t0 + n) (and their mirrors).
+__global__ void k_mm_zero_tail(MmGrid g, uint32_t t0, uint32_t ni, uint32_t nj, uint32_t* __restrict__ C,
+                               uint64_t ld) {
+    uint32_t tm, tn;
+    mm_tile_of(g, t0 + blockIdx.x, tm, tn);
+    for (uint32_t e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
+        const uint32_t row = tm * TILE + e / TILE, col = tn * TILE + e % TILE;
+        if (row < ni && col < nj) {
+            C[uint64_t(row) * ld + col] = 0;
+            if (g.self && tm != tn) C[uint64_t(col) * ld + row] = 0;
+        }
+    }
 }
 
 }  // namespace
@@ -169,33 +245,43 @@ int mm_tile_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint64
         if (!self) SI_TRY(build_bit_rows(ds, B.start, nj, w64, RB));
         return popcount_pairs(ctx, RA, ni, RB, nj, w64, false, d_C, ld);
     }
-    const uint64_t Kp = (K + KS - 1) / KS * KS;
+    // bit rows: w64 a multiple of the staged chunk, rows padded to the tile
+    const uint64_t w64 = (K + 64 * KC - 1) / (64 * KC) * KC;
     const uint64_t rA = (uint64_t(ni) + TILE - 1) / TILE * TILE, rB = (uint64_t(nj) + TILE - 1) / TILE * TILE;
-    const uint64_t bytes = (self ? rA : rA + rB) * Kp;
-    SI_TRY(ctx->ws_aux.ensure(bytes));
-    uint8_t* MA = ctx->ws_aux.as<uint8_t>();
-    uint8_t* MB = self ? MA : MA + rA * Kp;
-    const uint8_t one = variant == SI_MM_F32 ? 0x38 : 0x01;  // fp8 e4m3 1.0 / int8 1
-    SI_HIP(hipMemsetAsync(MA, 0, bytes, ctx->stream));
-    {
-        KernelScope ks(ctx, "mm_build");
-        hipLaunchKernelGGL(k_mm_build, dim3(4, std::min<uint32_t>(ni, 65535)), dim3(256), 0, ctx->stream, ds->d_elems,
-                           ds->d_offsets, A.start, ni, Kp, one, MA);
-        if (!self)
-            hipLaunchKernelGGL(k_mm_build, dim3(4, std::min<uint32_t>(nj, 65535)), dim3(256), 0, ctx->stream, ds->d_elems,
-                               ds->d_offsets, B.start, nj, Kp, one, MB);
-        SI_HIP(hipGetLastError());
-    }
+    const uint64_t rows = self ? rA : rA + rB;
+    SI_TRY(ctx->ws_aux.ensure(rows * w64 * 8));
+    auto* RA = ctx->ws_aux.as<unsigned long long>();
+    auto* RB = self ? RA : RA + rA * w64;
+    SI_HIP(hipMemsetAsync(RA, 0, rows * w64 * 8, ctx->stream));
+    SI_TRY(build_bit_rows(ds, A.start, ni, w64, RA, false));
+    if (!self) SI_TRY(build_bit_rows(ds, B.start, nj, w64, RB, false));
     const uint32_t tM = uint32_t(rA / TILE), tN = uint32_t(rB / TILE);
-    const uint64_t blocks = self ? uint64_t(tN) * (tN + 1) / 2 : uint64_t(tM) * tN;
-    if (blocks > 0x7FFFFFFFull) return fail(SI_ERR_RANGE, "too many GEMM tiles");
+    const uint64_t tiles = self ? uint64_t(tN) * (tN + 1) / 2 : uint64_t(tM) * tN;
+    if (tiles > 0x7FFFFFFFull) return fail(SI_ERR_RANGE, "too many GEMM tiles");
+    // whole tiles fill complete rounds of the resident slots (4 two-wave
+    // workgroups per CU); the tiles of a last partial round are split along K
+    const uint32_t slots = uint32_t(ctx->num_cus) * 4;
+    MmGrid g{uint32_t(tiles), 0, 1, tN, self};
+    const uint32_t nch = uint32_t(w64 / KC);
+    if (tiles > slots && tiles % slots) {
+        const uint32_t tail = uint32_t(tiles % slots);
+        uint32_t sp = 1;
+        while (sp * 2 * tail <= slots && sp * 2 <= 16 && sp * 2 <= nch) sp *= 2;
+        if (sp > 1) {
+            g.nfull = uint32_t(tiles) - tail;
+            g.ntail = tail;
+            g.nsplit = sp;
+            hipLaunchKernelGGL(k_mm_zero_tail, dim3(tail), dim3(256), 0, ctx->stream, g, g.nfull, ni, nj, d_C, ld);
+        }
+    }
+    const uint32_t units = g.nfull + g.ntail * g.nsplit;
+    const uint64_t* bA = reinterpret_cast<const uint64_t*>(RA);
+    const uint64_t* bB = reinterpret_cast<const uint64_t*>(RB);
     KernelScope ks(ctx, variant == SI_MM_F32 ? "mm_gemm_f32" : "mm_gemm_i8");
     if (variant == SI_MM_F32)
-        hipLaunchKernelGGL((k_mm_gemm<true>), dim3(uint32_t(blocks)), dim3(256), 0, ctx->stream, MA, ni, MB, nj, Kp, self,
-                           tN, d_C, ld);
+        hipLaunchKernelGGL((k_mm_bits<true>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
     else
-        hipLaunchKernelGGL((k_mm_gemm<false>), dim3(uint32_t(blocks)), dim3(256), 0, ctx->stream, MA, ni, MB, nj, Kp,
-                           self, tN, d_C, ld);
+        hipLaunchKernelGGL((k_mm_bits<false>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
     SI_HIP(hipGetLastError());
     return SI_OK;
 }

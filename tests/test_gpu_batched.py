@@ -93,14 +93,31 @@ def test_mm_ld_padding(ctx):
     assert np.array_equal(got[:, :30], exp)
 
 
-def test_mm_c4_shape_slice(ctx):
+@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32])
+def test_mm_c4_shape_slice(ctx, variant):
     """A 1024-set slice of the 8192 x 2^16 density workload (full K)."""
     coll = datasets.density_collection(1024, 1 << 16, 0.01, 0.10, 20261203)
     ds = ctx.upload(coll)
     A = si.tile(0, 0, 1024)
-    got = ds.mm_counts(A, A, si.MM_I8)
+    got = ds.mm_counts(A, A, variant)
     exp = oracle.tile_matrix(coll, 0, 1024, 0, 1024)
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32])
+def test_mm_split_tail(ctx, variant):
+    """6000 sets: 1128 self tiles, more than one round of resident workgroups,
+    so the last partial round is split along K and accumulated atomically.
+    Checked against the independent AND/popcount kernel everywhere and the
+    oracle on sampled rows."""
+    coll = datasets.density_collection(6000, 1 << 12, 0.01, 0.10, 77)
+    ds = ctx.upload(coll)
+    A = si.tile(0, 0, 6000)
+    got = ds.mm_counts(A, A, variant)
+    ref = ds.mm_counts(A, A, si.MM_POPCOUNT)
+    assert np.array_equal(got, ref)
+    for r0 in (0, 2990, 5936):
+        assert np.array_equal(got[r0:r0 + 64], oracle.tile_matrix(coll, r0, r0 + 64, 0, 6000)), r0
 
 
 # ------------------------------------------------------------------ edges --
