@@ -129,16 +129,16 @@ __global__ __launch_bounds__(NT) void k_bm_run(const uint32_t* __restrict__ base
 // --------------------------------------------------------- bitmap_naive --
 // Bitmaps of sets [s0, s0 + ns): row (s - s0) has W64 words of 64 bits.
 // Bit rows of sets [s0, s0 + ns): row s is w64 64-bit words (2 * w64 32-bit
-// words, same bit order) starting at rows + s * w64.  rows must be zero.
+// words, same bit order) starting at rows + s * w64.  rows must be zero.  A
+// workgroup takes whole sets (grid-stride); its waves split a set's chunks.
 __global__ void k_bm_build_rows(const uint32_t* __restrict__ base, const uint64_t* __restrict__ offsets, uint32_t s0,
                                 uint32_t ns, uint64_t w64, unsigned long long* __restrict__ rows) {
-    const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / WAVE);
-    for (uint32_t s = blockIdx.y; s < ns; s += gridDim.y) {
+    const uint32_t nw = blockDim.x / WAVE, w = threadIdx.x / WAVE;
+    for (uint32_t s = blockIdx.x; s < ns; s += gridDim.x) {
         const uint64_t o = offsets[s0 + s], n = offsets[s0 + s + 1] - o;
         uint32_t* row = reinterpret_cast<uint32_t*>(rows + uint64_t(s) * w64);
         const uint64_t chunks = (n + WAVE - 1) / WAVE;
-        for (uint64_t c = uint64_t(blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < chunks; c += waves)
-            or_bits_chunk(base + o, n, c, row);
+        for (uint64_t c = w; c < chunks; c += nw) or_bits_chunk(base + o, n, c, row);
     }
 }
 
@@ -289,8 +289,9 @@ int build_bit_rows(si_dataset* ds, uint32_t s0, uint32_t ns, uint64_t w64, unsig
     if (zero) SI_HIP(hipMemsetAsync(rows, 0, uint64_t(ns) * w64 * 8, ctx->stream));
     if (ns == 0) return SI_OK;
     KernelScope ks(ctx, "bitmap_build_rows");
-    dim3 grid(8, std::min<uint32_t>(ns, 65535));
-    hipLaunchKernelGGL(k_bm_build_rows, grid, dim3(NT), 0, ctx->stream, ds->d_elems, ds->d_offsets, s0, ns, w64, rows);
+    const uint32_t grid = std::min<uint32_t>(ns, uint32_t(ctx->num_cus) * 16);
+    hipLaunchKernelGGL(k_bm_build_rows, dim3(grid), dim3(NT), 0, ctx->stream, ds->d_elems, ds->d_offsets, s0, ns, w64,
+                       rows);
     SI_HIP(hipGetLastError());
     return SI_OK;
 }

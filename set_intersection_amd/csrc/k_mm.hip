@@ -13,10 +13,12 @@
 //    bytes the matrix cores consume are made in registers right before each
 //    MFMA, so the operand traffic is an eighth of a byte matrix's.
 //  * C = M_A · M_Bᵀ with v_mfma_i32_16x16x64_i8: a lane's 16 operand bytes
-//    are one row's 16 consecutive universe values, i.e. one 16-bit slice of
-//    that row's 64-bit word, expanded nibble by nibble
-//    ((n * 0x204081) & 0x01010101 puts bit i of n in byte i).  int32
-//    accumulation is exact below 2^31.
+//    are 16 universe values of one row, taken from a 32-bit word of its
+//    bitmap with one shift and one mask per 4 bytes (byte i of dword j is
+//    bit 8i + j of the shifted word); A and B use the same value-to-K
+//    assignment, so the products pair up.  int32 accumulation is exact
+//    below 2^31.  The K step is software-pipelined: the next fragment is
+//    expanded while the current fragment's MFMAs issue.
 //  * A two-wave workgroup computes a 128 x 128 tile (each wave 128 x 64,
 //    8 x 4 MFMA blocks: an expanded fragment feeds 4 or 8 MFMAs).  Chunks of
 //    8 words (512 universe values) of the 256 operand rows are staged through
@@ -47,15 +49,20 @@ constexpr int TILE = 128;
 constexpr int KC = 8;              // 64-bit words per staged chunk
 constexpr int RS = 2 * KC + 1;     // LDS row stride in dwords (odd)
 
-// 16 bits -> 16 bytes of 0/1 (int8) or 0/0x38 (fp8 1.0), byte i = bit i
+// A lane's 16 operand bytes: dword j (j = 0..3) of lane half p (p = 0, 1)
+// holds bits 8i + 4p + j (i = 0..3) of a 32-bit bitmap word in its bytes i,
+// as 0/1 (int8) or 0/0x38 (fp8 1.0).  Any fixed assignment of universe
+// values to MFMA K positions is correct as long as A and B use the same one;
+// this one costs a shift and a mask per dword.
 template <bool F8>
-__device__ __forceinline__ v4i expand16(uint32_t s16) {
+__device__ __forceinline__ v4i expand32(uint32_t x, uint32_t p) {
+    const uint32_t y = x >> (4 * p);
     v4i r;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        uint32_t v = __umul24(__builtin_amdgcn_ubfe(s16, 4 * d, 4), 0x204081u) & 0x01010101u;
+    for (int j = 0; j < 4; ++j) {
+        uint32_t v = (y >> j) & 0x01010101u;
         if (F8) v *= 0x38u;
-        r[d] = int(v);
+        r[j] = int(v);
     }
     return r;
 }
@@ -142,42 +149,81 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
-    // a lane's operand rows: 16 * i + (lane & 15); its 16-bit slice of a word
-    // is number (lane >> 4) (halfword 4 kk + q of the row's chunk)
-    const uint32_t r16 = lane & 15, q = lane >> 4;
-    const uint32_t rowA = r16 * RS, rowB = (wn * 64 + r16) * RS;
+    // a lane's operand rows: 16 * i + (lane & 15); lane quarter q takes half
+    // q & 1 of the bits of dword q >> 1 of each 64-bit word
+    const uint32_t r16 = lane & 15, q = lane >> 4, p = q & 1, h = q >> 1;
+    const uint32_t rowA = r16 * RS + h, rowB = (wn * 64 + r16) * RS + h;
     load(uint64_t(c0) * KC);
     store(0);
     __syncthreads();
     for (uint32_t c = c0; c < c1; ++c) {
         const uint32_t buf = (c - c0) & 1;
         if (c + 1 < c1) load(uint64_t(c + 1) * KC);
-        const uint16_t* hA = reinterpret_cast<const uint16_t*>(sA[buf]);
-        const uint16_t* hB = reinterpret_cast<const uint16_t*>(sB[buf]);
-        for (int kk = 0; kk < KC; ++kk) {
-            v4i fb[4];
+        const uint32_t* dA = sA[buf];
+        const uint32_t* dB = sB[buf];
+        if constexpr (!F32) {
+            // software pipeline over the 8 A fragments of a K step: while the
+            // 4 MFMAs of fragment i issue, fragment i+1 (and from i = 3 on one
+            // B fragment of the next K step) is expanded, so the expansion
+            // runs in the shadow of the matrix pipe
+            uint32_t wa[8], wb[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t sb = hB[2 * (rowB + j * 16 * RS) + 4 * kk + q];
-                fb[j] = expand16<F32>(sb);
-            }
+            for (int i = 0; i < 8; ++i) wa[i] = dA[rowA + i * 16 * RS];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const uint32_t sa = hA[2 * (rowA + i * 16 * RS) + 4 * kk + q];
-                const v4i fa = expand16<F32>(sa);
-                if constexpr (!F32) {
+            for (int j = 0; j < 4; ++j) wb[j] = dB[rowB + j * 16 * RS];
+            v4i fb[4], fbn[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) fb[j] = expand32<false>(wb[j], p);
+            v4i fan = expand32<false>(wa[0], p);
+#pragma unroll 2
+            for (int kk = 0; kk < KC; ++kk) {
+                const bool more = kk + 1 < KC;
+                uint32_t na[8], nb[4];
+                const int k2 = more ? 2 * (kk + 1) : 2 * kk;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) na[i] = dA[rowA + i * 16 * RS + k2];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) nb[j] = dB[rowB + j * 16 * RS + k2];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const v4i fa = fan;
+                    if (i < 7) fan = expand32<false>(wa[i + 1], p);
+                    else fan = expand32<false>(na[0], p);
+                    if (i >= 3 && i < 7) fbn[i - 3] = expand32<false>(nb[i - 3], p);
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[j], acc[i][j], 0, 0, 0);
-                } else {
+                    // interleave: one MFMA, then a few VALU of the expansions
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                        if (i >= 3 && i < 7)
+                            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+                        else
+                            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) wa[i] = na[i];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fb[j] = fbn[j];
+            }
+        } else {
+            for (int kk = 0; kk < KC; ++kk) {
+                v4i fb[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fb[j] = expand32<true>(dB[rowB + j * 16 * RS + 2 * kk], p);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const v4i fa = expand32<true>(dA[rowA + i * 16 * RS + 2 * kk], p);
                     // the fp8 MFMA takes 8 bytes per lane: the fragment's two
                     // halves are two K sub-steps (the same for A and B)
 #pragma unroll
-                    for (int h = 0; h < 2; ++h)
+                    for (int hh = 0; hh < 2; ++hh)
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const long av = (long(uint32_t(fa[2 * h + 1])) << 32) | uint32_t(fa[2 * h]);
-                            const long bv = (long(uint32_t(fb[j][2 * h + 1])) << 32) | uint32_t(fb[j][2 * h]);
+                            const long av = (long(uint32_t(fa[2 * hh + 1])) << 32) | uint32_t(fa[2 * hh]);
+                            const long bv = (long(uint32_t(fb[j][2 * hh + 1])) << 32) | uint32_t(fb[j][2 * hh]);
                             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, bv, acc[i][j], 0, 0, 0);
                         }
                 }
