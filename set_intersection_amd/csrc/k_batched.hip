@@ -316,20 +316,107 @@ __global__ void k_gather(const uint32_t* __restrict__ elems, const uint64_t* __r
 }
 
 
-// Edge lists: items (row u, partner range [p0, p1) of plist); every partner
-// is one (set id, output index) pair.  Waves split the item's partners and
-// walk them in batches of 64: the batch's partner sets are concatenated
-// virtually (lane t owns [excl_t, incl_t)), each element position finds its
-// partner by a shuffle binary search, and each lane sums the hit bits of its
-// own range.  Each partner is handled by exactly one wave.
+// ------------------------------------------------------------ edge lists --
+// A partner record is uint4 {element offset low, high, inclusive end of the
+// partner's elements within its item, output index}.  Items are (row,
+// partner range [j0, j1), mode); the item's partner elements are numbered
+// 0 .. end[j1-1] in partner order (a virtual concatenation).
+//
+// Lanes hold 64 consecutive partners of a wave's element range; every
+// element position finds its partner by a shuffle binary search over the
+// lanes' start positions, is probed against the row structure, and each lane
+// sums the hit bits of its own partner's positions (ballot + popcount).  A
+// partner wholly inside [ws, we) gets its count stored; one cut by ws or we
+// leaves its part in cutSet/cutCnt (slot 0: began before ws, slot 1: goes on
+// past we).  Partners must have at least one element.
+
+template <typename ProbeFn>
+__device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uint32_t j0, uint32_t j1, uint32_t ws,
+                                             uint32_t we, const uint32_t* __restrict__ elems, ProbeFn probeFn,
+                                             uint32_t* __restrict__ out, uint32_t* cutSet, uint32_t* cutCnt) {
+    const uint32_t lane = lane_id();
+    // first partner whose elements end past ws
+    uint32_t lo = j0, hi = j1 - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (part[mid].z > ws) hi = mid; else lo = mid + 1;
+    }
+    uint32_t qc = lo, carry = 0;
+    uint32_t c0 = ws;
+    while (c0 < we) {
+        const uint32_t q = qc + lane;
+        const bool valid = q < j1;
+        const uint4 P = valid ? part[q] : make_uint4(0, 0, 0xFFFFFFFFu, 0);
+        const uint32_t end = P.z;
+        uint32_t start = __shfl_up(end, 1, WAVE);
+        const uint32_t start0 = (qc == j0) ? 0u : part[qc - 1].z;
+        if (lane == 0) start = start0;
+        const uint32_t e63 = __shfl(end, WAVE - 1, WAVE);
+        uint32_t c1 = c0 + WAVE * VT;
+        c1 = c1 < we ? c1 : we;
+        c1 = c1 < e63 ? c1 : e63;
+        const uint64_t off = (uint64_t(P.y) << 32) | P.x;
+        uint64_t M[VT];
+#pragma unroll
+        for (int v = 0; v < VT; ++v) {
+            const uint32_t pos = c0 + uint32_t(v) * WAVE + lane;
+            // largest lane r with start_r <= pos (every lane takes part)
+            int rl = 0, rh = WAVE - 1;
+#pragma unroll
+            for (int step = 0; step < 6; ++step) {
+                const int mid = (rl + rh + 1) >> 1;
+                if (__shfl(start, mid, WAVE) <= pos) rl = mid; else rh = mid - 1;
+            }
+            const uint32_t sr = __shfl(start, rl, WAVE);
+            const uint64_t orr = __shfl(off, rl, WAVE);
+            uint32_t hit = 0;
+            if (pos < c1) hit = probeFn(elems[orr + (pos - sr)]);
+            M[v] = __ballot(hit);
+        }
+        // this lane's partner covers [start, end); its positions in [c0, c1)
+        uint32_t cnt = 0;
+        const bool touches = valid && start < c1 && end > c0;
+        if (touches) {
+            const int64_t a = int64_t(start > c0 ? start : c0) - int64_t(c0);
+            const int64_t b = int64_t(end < c1 ? end : c1) - int64_t(c0);
+#pragma unroll
+            for (int v = 0; v < VT; ++v) {
+                int l2 = int(a - 64 * v), h2 = int(b - 64 * v);
+                l2 = l2 < 0 ? 0 : (l2 > 64 ? 64 : l2);
+                h2 = h2 < 0 ? 0 : (h2 > 64 ? 64 : h2);
+                if (h2 > l2) cnt += __popcll(M[v] & bit_range(l2, h2));
+            }
+            if (lane == 0) cnt += carry;
+            if (end <= c1) {
+                if (start >= ws && end <= we) out[P.w] = cnt;                  // whole partner in this wave
+                else if (cutSet) { cutSet[0] = q; cutCnt[0] = cnt; }         // began before ws
+            } else if (c1 == we && cutSet) {
+                cutSet[1] = q;                                                // continues past we
+                cutCnt[1] = cnt;
+            }
+        }
+        // next batch starts at the first partner not finished by c1
+        const uint64_t done = __ballot(valid && end <= c1);
+        const uint32_t nd = uint32_t(__popcll(done));
+        const uint32_t contCnt = __shfl(cnt, nd < WAVE ? nd : 0, WAVE);
+        carry = (nd < WAVE) ? contCnt : 0u;
+        qc += nd;
+        c0 = c1;
+    }
+}
+
+// Rows whose structure is shared by a workgroup (BITMAP, HASH, SEARCH):
+// persistent workgroups pop items; the waves split the item's partner
+// elements evenly; partners cut by wave boundaries are completed through LDS.
 template <int NT, int LDSW>
-__global__ __launch_bounds__(NT) void k_rows_list(const Item* __restrict__ items, uint32_t nitems,
-                                                  uint32_t* __restrict__ queue, const uint32_t* __restrict__ elems,
-                                                  const uint64_t* __restrict__ offsets, const uint2* __restrict__ plist,
-                                                  uint32_t bmWords, uint32_t* __restrict__ out) {
+__global__ __launch_bounds__(NT) void k_edges_wg(const Item* __restrict__ items, uint32_t nitems,
+                                                 uint32_t* __restrict__ queue, const uint32_t* __restrict__ elems,
+                                                 const uint64_t* __restrict__ offsets, const uint4* __restrict__ part,
+                                                 uint32_t bmWords, uint32_t* __restrict__ out) {
     constexpr int NW = NT / WAVE;
+    static_assert(NW <= 32, "partial slots");
     __shared__ RowSmem<LDSW> sm;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+    const uint32_t tid = threadIdx.x, w = tid / WAVE;
     if (tid == 0) {
         sm.row = EMPTY;
         sm.mode = EMPTY;
@@ -337,6 +424,7 @@ __global__ __launch_bounds__(NT) void k_rows_list(const Item* __restrict__ items
     __syncthreads();
     while (true) {
         if (tid == 0) sm.item = atomicAdd(queue, 1u);
+        if (tid < 2 * NW) sm.pset[tid] = EMPTY;
         __syncthreads();
         const uint32_t it = sm.item;
         if (it >= nitems) break;
@@ -352,61 +440,96 @@ __global__ __launch_bounds__(NT) void k_rows_list(const Item* __restrict__ items
         }
         const uint32_t mode = I.mode, logTS = sm.logTS, hasMax = sm.hasMax;
         const uint32_t* S = elems + rowOff;
-        const uint32_t np = I.j1 - I.j0;
-        const uint32_t pa = I.j0 + uint32_t((uint64_t(np) * w) / NW), pb = I.j0 + uint32_t((uint64_t(np) * (w + 1)) / NW);
-        for (uint32_t q0 = pa; q0 < pb; q0 += WAVE) {
-            const uint32_t q = q0 + lane;
-            const bool valid = q < pb;
-            uint2 pe = make_uint2(0, 0);
-            uint64_t off = 0, sz = 0;
-            if (valid) {
-                pe = plist[q];
-                off = offsets[pe.x];
-                sz = offsets[pe.x + 1] - off;
-            }
-            uint64_t incl = sz;
-#pragma unroll
-            for (int d = 1; d < WAVE; d <<= 1) {
-                const uint64_t t = __shfl_up(incl, d, WAVE);
-                if (lane >= uint32_t(d)) incl += t;
-            }
-            const uint64_t excl = incl - sz;
-            const uint64_t tot = __shfl(incl, WAVE - 1, WAVE);
-            uint32_t cnt = 0;
-            for (uint64_t c0 = 0; c0 < tot; c0 += WAVE * VT) {
-                uint64_t M[VT];
-#pragma unroll
-                for (int v = 0; v < VT; ++v) {
-                    const uint64_t pos = c0 + uint64_t(v) * WAVE + lane;
-                    uint32_t hit = 0;
-                    // largest r with excl_r <= pos (all lanes take part in the shuffles)
-                    int lo = 0, hi = WAVE - 1;
-#pragma unroll
-                    for (int step = 0; step < 6; ++step) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        const uint64_t em = __shfl(excl, mid, WAVE);
-                        if (em <= pos) lo = mid; else hi = mid - 1;
+        const uint32_t T = part[I.j1 - 1].z;
+        const uint32_t ws = uint32_t((uint64_t(T) * w) / NW), we = uint32_t((uint64_t(T) * (w + 1)) / NW);
+        if (ws < we)
+            edge_batches(part, I.j0, I.j1, ws, we, elems,
+                         [&](uint32_t x) { return probe<LDSW>(sm, mode, x, S, rowN, logTS, hasMax); }, out,
+                         &sm.pset[2 * w], &sm.pcnt[2 * w]);
+        __syncthreads();
+        if (tid == 0) {
+            for (int s = 0; s < 2 * NW; ++s) {
+                const uint32_t j = sm.pset[s];
+                if (j == EMPTY) continue;
+                uint32_t tot = 0;
+                for (int t = s; t < 2 * NW; ++t)
+                    if (sm.pset[t] == j) {
+                        tot += sm.pcnt[t];
+                        if (t != s) sm.pset[t] = EMPTY;
                     }
-                    const uint64_t er = __shfl(excl, lo, WAVE);
-                    const uint64_t orr = __shfl(off, lo, WAVE);
-                    if (pos < tot) hit = probe<LDSW>(sm, mode, elems[orr + (pos - er)], S, rowN, logTS, hasMax);
-                    M[v] = __ballot(hit);
-                }
-                if (valid && sz) {
-                    const int64_t a = int64_t(excl) - int64_t(c0), b = int64_t(incl) - int64_t(c0);
-#pragma unroll
-                    for (int v = 0; v < VT; ++v) {
-                        int64_t l2 = a - 64 * v, h2 = b - 64 * v;
-                        l2 = l2 < 0 ? 0 : (l2 > 64 ? 64 : l2);
-                        h2 = h2 < 0 ? 0 : (h2 > 64 ? 64 : h2);
-                        if (h2 > l2) cnt += __popcll(M[v] & bit_range(int(l2), int(h2)));
-                    }
-                }
+                out[part[j].w] = tot;
             }
-            if (valid) out[pe.y] = cnt;
         }
         __syncthreads();
     }
+}
+
+// Small rows (at most WAVE_HASH_MAX elements): every wave pops its own items
+// and keeps the row's open-addressing table in a private LDS region, so an
+// item costs no workgroup barrier.
+constexpr uint32_t WAVE_HASH_MAX = 511;
+constexpr int WH_NT = 256, WH_SLOTS = 1024;
+
+__global__ __launch_bounds__(WH_NT) void k_edges_wave(const Item* __restrict__ items, uint32_t nitems,
+                                                      uint32_t* __restrict__ queue, const uint32_t* __restrict__ elems,
+                                                      const uint64_t* __restrict__ offsets,
+                                                      const uint4* __restrict__ part, uint32_t* __restrict__ out) {
+    __shared__ uint32_t tabs[WH_NT / WAVE][WH_SLOTS];
+    const uint32_t lane = lane_id();
+    uint32_t* tab = tabs[threadIdx.x / WAVE];
+    while (true) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(queue, 1u);
+        it = __shfl(it, 0, WAVE);
+        if (it >= nitems) break;
+        const Item I = items[it];
+        const uint64_t ro = offsets[I.row];
+        const uint32_t rn = uint32_t(offsets[I.row + 1] - ro);
+        uint32_t logTS = 6;
+        while ((1u << logTS) < 2 * rn) ++logTS;
+        const uint32_t TS = 1u << logTS, mask = TS - 1;
+        for (uint32_t i = lane; i < TS; i += WAVE) tab[i] = EMPTY;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        bool mx = false;
+        for (uint32_t i = lane; i < rn; i += WAVE) {
+            const uint32_t x = elems[ro + i];
+            if (x == EMPTY) {
+                mx = true;
+                continue;
+            }
+            uint32_t sl = hslot(x, logTS);
+            while (true) {
+                const uint32_t prev = atomicCAS(&tab[sl], EMPTY, x);
+                if (prev == EMPTY || prev == x) break;
+                sl = (sl + 1) & mask;
+            }
+        }
+        const uint32_t hasMax = __ballot(mx) ? 1u : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t T = part[I.j1 - 1].z;
+        if (T > 0)
+            edge_batches(part, I.j0, I.j1, 0, T, elems,
+                         [&](uint32_t x) -> uint32_t {
+                             if (x == EMPTY) return hasMax;
+                             uint32_t sl = hslot(x, logTS);
+                             while (true) {
+                                 const uint32_t k = tab[sl];
+                                 if (k == x) return 1u;
+                                 if (k == EMPTY) return 0u;
+                                 sl = (sl + 1) & mask;
+                             }
+                         },
+                         out, nullptr, nullptr);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// out[idx[i]] = 0 for partners without elements
+__global__ void k_zero_list(const uint32_t* __restrict__ idx, uint32_t n, uint32_t* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[idx[i]] = 0;
 }
 
 template <int NT>
@@ -735,15 +858,17 @@ struct si_edge_plan {
     si_dataset* ds = nullptr;
     uint64_t m = 0;
     si_tuning t{};
-    Item* d_bm = nullptr;
-    Item* d_pr = nullptr;
-    uint2* d_plist = nullptr;
+    Item* d_wg = nullptr;    // BITMAP / HASH / SEARCH rows (workgroup items)
+    Item* d_wave = nullptr;  // small HASH rows (wave items)
+    uint4* d_part = nullptr;
     uint4* d_merge = nullptr;
+    uint32_t* d_zero = nullptr;
     uint32_t* d_queue = nullptr;
     uint64_t* d_partials = nullptr;
-    uint32_t n_bm = 0, n_pr = 0, n_merge = 0;
+    uint32_t n_bm = 0, n_wg = 0, n_wave = 0, n_merge = 0, n_zero = 0;
     ~si_edge_plan() {
-        for (void* p : {(void*)d_bm, (void*)d_pr, (void*)d_plist, (void*)d_merge, (void*)d_queue, (void*)d_partials})
+        for (void* p : {(void*)d_wg, (void*)d_wave, (void*)d_part, (void*)d_merge, (void*)d_zero, (void*)d_queue,
+                        (void*)d_partials})
             if (p) (void)hipFree(p);
     }
 };
@@ -770,6 +895,7 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
     plan->m = m;
     plan->t = t;
     const auto& sz = ds->sizes;
+    const auto& of = ds->offsets;
     // row = endpoint with the larger list (ties: the smaller id); counting sort by row
     std::vector<uint32_t> rowOf(m), cnt(size_t(k) + 1, 0);
     for (uint64_t e = 0; e < m; ++e) {
@@ -779,7 +905,7 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
         cnt[r + 1]++;
     }
     for (uint32_t i = 0; i < k; ++i) cnt[i + 1] += cnt[i];
-    std::vector<uint2> plist(m);
+    std::vector<uint2> plist(m);  // (partner set, edge index) grouped by row
     {
         std::vector<uint32_t> pos(cnt.begin(), cnt.end() - 1);
         for (uint64_t e = 0; e < m; ++e) {
@@ -788,40 +914,67 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
         }
     }
     const bool bmFits = bitmap_words(ds) <= uint32_t(BM_WORDS);
-    std::vector<Item> bm, pr;
-    std::vector<uint4> merge;
-    std::vector<uint2> plist2;  // final partner order (SEARCH/probe partners of BIG rows first-class)
-    plist2.reserve(m);
+    std::vector<Item> bmItems, wgItems, waveItems;
+    std::vector<uint4> part, merge;
+    std::vector<uint32_t> zero;
+    part.reserve(m);
+    // partners [p0, p1) of `part` (already appended) cut into items of about
+    // `chunk` partner elements; part[q].z becomes the inclusive end within its item
     auto emit = [&](std::vector<Item>& vec, uint32_t row, uint32_t p0, uint32_t p1, uint32_t mode, uint64_t chunk) {
         uint32_t p = p0;
         while (p < p1) {
             uint64_t acc = 0;
             uint32_t e = p;
-            while (e < p1 && (e == p || acc + sz[plist2[e].x] <= chunk)) acc += sz[plist2[e++].x];
+            while (e < p1 && (e == p || acc + part[e].z <= chunk)) acc += part[e++].z;
+            uint32_t run = 0;
+            for (uint32_t q = p; q < e; ++q) {
+                run += part[q].z;
+                part[q].z = run;
+            }
             vec.push_back(Item{row, p, e, mode});
             p = e;
         }
     };
+    auto add = [&](const uint2& pe) {
+        const uint32_t n = sz[pe.x];
+        if (n == 0) {
+            zero.push_back(pe.y);
+            return;
+        }
+        const uint64_t o = of[pe.x];
+        part.push_back(make_uint4(uint32_t(o), uint32_t(o >> 32), n, pe.y));  // .z: size until emit
+    };
     for (uint32_t r = 0; r < k; ++r) {
         if (cnt[r] == cnt[r + 1]) continue;
         const uint32_t s = sz[r];
-        const uint32_t start = uint32_t(plist2.size());
-        if ((bmFits && (s >= t.bitmap_min || s > HASH_MAX)) || s <= HASH_MAX) {
-            for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) plist2.push_back(plist[q]);
-            const bool useBm = bmFits && (s >= t.bitmap_min || s > HASH_MAX);
+        const uint32_t start = uint32_t(part.size());
+        const bool useBm = bmFits && (s >= t.bitmap_min || s > HASH_MAX);
+        if (s == 0) {
+            for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) zero.push_back(plist[q].y);
+            continue;
+        }
+        if (useBm || s <= HASH_MAX) {
+            for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) add(plist[q]);
+            const uint32_t end = uint32_t(part.size());
+            if (end == start) continue;
             if (useBm)
-                emit(bm, r, start, uint32_t(plist2.size()), MODE_BITMAP, t.chunk_elems_bitmap);
+                emit(bmItems, r, start, end, MODE_BITMAP, t.chunk_elems_bitmap);
+            else if (s <= WAVE_HASH_MAX)
+                emit(waveItems, r, start, end, MODE_HASH, t.chunk_elems);
             else
-                emit(pr, r, start, uint32_t(plist2.size()), MODE_HASH, t.chunk_elems);
+                emit(wgItems, r, start, end, MODE_HASH, t.chunk_elems);
         } else {
             for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) {
                 const uint2 pe = plist[q];
-                if (uint64_t(sz[pe.x]) * t.ratio_search > s)
+                if (sz[pe.x] == 0)
+                    zero.push_back(pe.y);
+                else if (uint64_t(sz[pe.x]) * t.ratio_search > s)
                     merge.push_back(make_uint4(r, pe.x, pe.y, 0));
                 else
-                    plist2.push_back(pe);
+                    add(pe);
             }
-            if (plist2.size() > start) emit(pr, r, start, uint32_t(plist2.size()), MODE_SEARCH, t.chunk_elems);
+            const uint32_t end = uint32_t(part.size());
+            if (end > start) emit(wgItems, r, start, end, MODE_SEARCH, t.chunk_elems);
         }
     }
     si_context* ctx = ds->ctx;
@@ -832,17 +985,22 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
             SI_HIP(hipMemcpyAsync(dptr, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
         return SI_OK;
     };
-    SI_TRY(up(bm, plan->d_bm));
-    SI_TRY(up(pr, plan->d_pr));
-    SI_TRY(up(plist2, plan->d_plist));
+    // workgroup items: BITMAP first (launched with the 1024-thread kernel), then the rest
+    plan->n_bm = uint32_t(bmItems.size());
+    bmItems.insert(bmItems.end(), wgItems.begin(), wgItems.end());
+    SI_TRY(up(bmItems, plan->d_wg));
+    SI_TRY(up(waveItems, plan->d_wave));
+    SI_TRY(up(part, plan->d_part));
     SI_TRY(up(merge, plan->d_merge));
+    SI_TRY(up(zero, plan->d_zero));
     SI_HIP(hipMalloc(&plan->d_queue, 256));
     SI_HIP(hipMemsetAsync(plan->d_queue, 0, 256, ctx->stream));
     SI_HIP(hipMalloc(&plan->d_partials, SUM_GRID * 8));
     SI_HIP(hipStreamSynchronize(ctx->stream));
-    plan->n_bm = uint32_t(bm.size());
-    plan->n_pr = uint32_t(pr.size());
+    plan->n_wg = uint32_t(wgItems.size());
+    plan->n_wave = uint32_t(waveItems.size());
     plan->n_merge = uint32_t(merge.size());
+    plan->n_zero = uint32_t(zero.size());
     ds->ctx->edge_plans.insert(plan.get());
     *out = plan.release();
     return SI_OK;
@@ -852,17 +1010,27 @@ int edge_count(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
     si_dataset* ds = P->ds;
     si_context* ctx = ds->ctx;
     hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, ctx->stream, P->d_queue);
+    if (P->n_zero) {
+        const uint32_t grid = uint32_t(std::min<uint64_t>((P->n_zero + 255) / 256, 1024));
+        hipLaunchKernelGGL(k_zero_list, dim3(grid), dim3(256), 0, ctx->stream, P->d_zero, P->n_zero, counts);
+    }
     if (P->n_bm) {
         KernelScope ks(ctx, "edges_rows_bitmap");
         const uint32_t grid = uint32_t(std::min<uint64_t>(ctx->num_cus, P->n_bm));
-        hipLaunchKernelGGL((k_rows_list<BM_NT, BM_WORDS>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_bm, P->n_bm,
-                           P->d_queue + 0, ds->d_elems, ds->d_offsets, P->d_plist, bitmap_words(ds), counts);
+        hipLaunchKernelGGL((k_edges_wg<BM_NT, BM_WORDS>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_wg, P->n_bm,
+                           P->d_queue + 0, ds->d_elems, ds->d_offsets, P->d_part, bitmap_words(ds), counts);
     }
-    if (P->n_pr) {
+    if (P->n_wg) {
         KernelScope ks(ctx, "edges_rows_probe");
-        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * P->t.probe_wg_per_cu, P->n_pr));
-        hipLaunchKernelGGL((k_rows_list<PR_NT, PR_WORDS>), dim3(grid), dim3(PR_NT), 0, ctx->stream, P->d_pr, P->n_pr,
-                           P->d_queue + 1, ds->d_elems, ds->d_offsets, P->d_plist, bitmap_words(ds), counts);
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * P->t.probe_wg_per_cu, P->n_wg));
+        hipLaunchKernelGGL((k_edges_wg<PR_NT, PR_WORDS>), dim3(grid), dim3(PR_NT), 0, ctx->stream, P->d_wg + P->n_bm,
+                           P->n_wg, P->d_queue + 1, ds->d_elems, ds->d_offsets, P->d_part, bitmap_words(ds), counts);
+    }
+    if (P->n_wave) {
+        KernelScope ks(ctx, "edges_rows_wave");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * 8, (P->n_wave + 3) / 4));
+        hipLaunchKernelGGL(k_edges_wave, dim3(grid), dim3(WH_NT), 0, ctx->stream, P->d_wave, P->n_wave, P->d_queue + 3,
+                           ds->d_elems, ds->d_offsets, P->d_part, counts);
     }
     if (P->n_merge) {
         KernelScope ks(ctx, "edges_merge_pairs");
