@@ -34,18 +34,15 @@ using namespace si_dev;
 
 // ------------------------------------------------------- bitmap_dynamic --
 // Sets bit x of the 32-bit-word bitmap bm for the elements x of the sorted
-// unique list S[c*64, c*64+64) handled by this wave (chunk c).  Elements of
-// one word are consecutive, so lanes that share a word OR their bits together
-// (segmented OR over the wave with shuffles) and the segment's first lane
-// writes the word: a plain store inside the chunk, an atomic OR only for the
-// chunk's first and last word, which the neighbouring chunks may share.
-// bm must be zero beforehand.
-__device__ __forceinline__ void or_bits_chunk(const uint32_t* __restrict__ S, uint64_t n, uint64_t c,
-                                              uint32_t* __restrict__ bm) {
+// unique list S[c*64, c*64+64) handled by this wave (chunk c; x is this lane's
+// element, valid when c*64 + lane < n).  Elements of one word are
+// consecutive, so lanes that share a word OR their bits together (segmented
+// OR over the wave with shuffles) and the segment's first lane writes the
+// word: a plain store inside the chunk, an atomic OR only for the chunk's
+// first and last word, which the neighbouring chunks may share.  bm must be
+// zero beforehand.
+__device__ __forceinline__ void or_bits_chunk(uint32_t x, bool valid, uint32_t* __restrict__ bm) {
     const uint32_t lane = lane_id();
-    const uint64_t i = c * WAVE + lane;
-    const bool valid = i < n;
-    const uint32_t x = valid ? S[i] : 0u;
     const uint32_t w = valid ? (x >> 5) : 0xFFFFFFFFu;
     uint32_t bits = valid ? (1u << (x & 31)) : 0u;
 #pragma unroll
@@ -64,11 +61,33 @@ __device__ __forceinline__ void or_bits_chunk(const uint32_t* __restrict__ S, ui
     }
 }
 
+// Chunks cb, cb + stride, ... (UNROLL of them) of S[0, n): all loads are
+// issued before the first chunk is combined, so a wave keeps UNROLL chunk
+// loads in flight.
+constexpr int BUILD_UNROLL = 8;
+
+__device__ __forceinline__ void or_bits_chunks(const uint32_t* __restrict__ S, uint64_t n, uint64_t cb,
+                                               uint64_t stride, uint32_t* __restrict__ bm) {
+    const uint32_t lane = lane_id();
+    uint32_t x[BUILD_UNROLL];
+    bool v[BUILD_UNROLL];
+#pragma unroll
+    for (int u = 0; u < BUILD_UNROLL; ++u) {
+        const uint64_t i = (cb + uint64_t(u) * stride) * WAVE + lane;
+        v[u] = i < n;
+        x[u] = v[u] ? S[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < BUILD_UNROLL; ++u)
+        if (cb + uint64_t(u) * stride < (n + WAVE - 1) / WAVE) or_bits_chunk(x[u], v[u], bm);
+}
+
 __global__ void k_bm_build(const uint32_t* __restrict__ S, uint64_t n, uint32_t* __restrict__ bm) {
     const uint64_t chunks = (n + WAVE - 1) / WAVE;
     const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / WAVE);
-    for (uint64_t c = uint64_t(blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < chunks; c += waves)
-        or_bits_chunk(S, n, c, bm);
+    for (uint64_t c = uint64_t(blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < chunks;
+         c += waves * BUILD_UNROLL)
+        or_bits_chunks(S, n, c, waves, bm);
 }
 
 template <int NT>
@@ -138,7 +157,7 @@ __global__ void k_bm_build_rows(const uint32_t* __restrict__ base, const uint64_
         const uint64_t o = offsets[s0 + s], n = offsets[s0 + s + 1] - o;
         uint32_t* row = reinterpret_cast<uint32_t*>(rows + uint64_t(s) * w64);
         const uint64_t chunks = (n + WAVE - 1) / WAVE;
-        for (uint64_t c = w; c < chunks; c += nw) or_bits_chunk(base + o, n, c, row);
+        for (uint64_t c = w; c < chunks; c += uint64_t(nw) * BUILD_UNROLL) or_bits_chunks(base + o, n, c, nw, row);
     }
 }
 
