@@ -64,7 +64,7 @@ constexpr uint64_t kInclusive = 1ull << 62;
 // the total to out; the last workgroup to finish clears state and resets
 // done[0] and done[1].
 template <int NT, int VT, int WN, bool MAT, bool SEL>
-__global__ __launch_bounds__(NT) void k_sisi_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sisi_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
                                                     uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
                                                     uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
                                                     uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
@@ -94,8 +94,7 @@ __global__ __launch_bounds__(NT) void k_sisi_stream(const uint32_t* __restrict__
     const uint64_t ib = dBeg - ia, ibE = dEnd - iaE;
 
     uint32_t* slot = MAT ? stage + uint64_t(s) * stage_cap : nullptr;
-    const uint64_t c = merge_segment_stream<NT, VT, WN, MAT, SEL>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm, slot,
-                                                                  stage_cap);
+    const uint64_t c = merge_segment_stream<NT, VT, WN, MAT, SEL>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm, slot);
     const uint64_t total = block_sum_u64<NT>(c, sm.red);
 
     if (MAT) {

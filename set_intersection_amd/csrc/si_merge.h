@@ -425,12 +425,10 @@ __device__ __forceinline__ uint32_t split_search(const uint32_t* wa, uint32_t sa
 
 // SEL: one LDS read per merge step (the next element of the side just
 // consumed, selected by address) instead of re-reading both heads.
-// With MAT, stage_cap is the staging slot's capacity in elements.
 template <int NT, int VT, int WN, bool MAT, bool SEL = false>
 __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
                                          uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
-                                         StreamSmem<NT, VT, WN>& sm, uint32_t* __restrict__ stage,
-                                         uint64_t stage_cap = 0) {
+                                         StreamSmem<NT, VT, WN>& sm, uint32_t* __restrict__ stage) {
     using C = StreamCfg<NT, VT, WN>;
     constexpr uint32_t T = C::T, W = C::W, M = C::M;
     const uint32_t tid = threadIdx.x;
@@ -513,8 +511,9 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
         // 3. merge this thread's steps; A is taken on ties and matched against
         //    the current B head (which may lie past y1)
         uint32_t cnt = 0;
-        uint32_t mval[MAT ? VT : 1];
-        uint32_t mbits = 0;  // MAT: bit (VT-1-k) set when step k is a match
+        // MAT: bit (VT-1-k) of mbits is set when step k is a match, of tbits
+        // when step k takes an A element
+        uint32_t mbits = 0, tbits = 0;
         const uint32_t steps = de - ds;
         if (aMore && bMore && steps == uint32_t(VT)) {
             // x1 <= W-1 and y1 <= W-1: every head read is a true element and
@@ -531,8 +530,8 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
                 const bool takeA = av <= bv;
                 const bool hit = av == bv;
                 if (MAT) {
-                    mval[k] = av;
                     mbits = 2 * mbits + uint32_t(hit);
+                    tbits = 2 * tbits + uint32_t(takeA);
                 } else {
                     hits += uint32_t(__popcll(__ballot(hit)));
                 }
@@ -561,11 +560,11 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
             uint32_t bv = y < nBw ? rB[(sb + y) & M] : 0u;
 #pragma unroll
             for (int k = 0; k < VT; ++k) {
-                bool hit = false;
+                bool hit = false, took = false;
                 if (uint32_t(k) < steps) {
                     const bool takeA = (x < x1) && (y >= y1 || av <= bv);
                     hit = takeA && (y < nBw) && (av == bv);
-                    if (MAT) mval[k] = av;
+                    took = takeA;
                     if (takeA) {
                         ++x;
                         if (x < x1) av = rA[(sa + x) & M];
@@ -575,7 +574,10 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
                     }
                 }
                 cnt += uint32_t(hit);
-                if (MAT) mbits = 2 * mbits + uint32_t(hit);
+                if (MAT) {
+                    mbits = 2 * mbits + uint32_t(hit);
+                    tbits = 2 * tbits + uint32_t(took);
+                }
             }
         }
         myCount += cnt;
@@ -586,10 +588,11 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
             excl = incl - cnt;
             if (lane == WAVE - 1) sm.wave_tot[tid / WAVE] = incl;
         }
-        __syncthreads();  // all window reads of this tile are done
+        __syncthreads();  // the tile's merge reads are done; wave totals are published
 
-        // 4. commit the refill; with MAT, stage this tile's matches
-        r.commit(sm);
+        // 4. with MAT, stage this tile's matches: the matched A elements are
+        //    still in the window (the refill lands after the next barrier);
+        //    step k's A element is x0 + (A steps before k)
         if (MAT) {
             const uint32_t w = tid / WAVE;
             uint32_t before = 0;
@@ -600,12 +603,19 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
                 wtot += t;
             }
             uint32_t* dst = stage + written + before + excl;
-#pragma unroll
-            for (int k = 0; k < VT; ++k) {
-                if (mbits & (1u << (VT - 1 - k))) *dst++ = mval[k];
+            uint32_t mb = mbits;
+            while (mb) {
+                const uint32_t hb = 31u - uint32_t(__builtin_clz(mb));  // earliest remaining match
+                const uint32_t j = uint32_t(__popc(tbits >> (hb + 1)));
+                *dst++ = rA[(sa + x0 + j) & M];
+                mb &= ~(1u << hb);
             }
             written += wtot;
+            __syncthreads();  // staged values are read before the refill overwrites their slots
         }
+
+        // 5. commit the refill
+        r.commit(sm);
         ia += ca;
         ib += cb;
         d += Tc;
