@@ -135,8 +135,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
         }
         __syncthreads();
+        // place the slot: eight loads in flight per thread
         const uint64_t pre = shPrefix;
-        for (uint64_t i = threadIdx.x; i < total; i += NT) list[pre + i] = slot[i];
+        constexpr int CU = 8;
+        for (uint64_t i0 = threadIdx.x; i0 < total; i0 += uint64_t(NT) * CU) {
+            uint32_t v[CU];
+#pragma unroll
+            for (int u = 0; u < CU; ++u) {
+                const uint64_t i = i0 + uint64_t(u) * NT;
+                v[u] = i < total ? slot[i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < CU; ++u) {
+                const uint64_t i = i0 + uint64_t(u) * NT;
+                if (i < total) list[pre + i] = v[u];
+            }
+        }
         if (threadIdx.x == 0) shTicket = atomicAdd(&done[0], 1u);
         __syncthreads();
         if (shTicket == nseg - 1) {  // every segment has finished its look-back
