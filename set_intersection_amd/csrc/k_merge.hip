@@ -70,7 +70,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                     uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
                                                     uint32_t* __restrict__ stage, uint64_t stage_cap,
                                                     uint32_t* __restrict__ list, uint64_t* __restrict__ state) {
-    __shared__ StreamSmem<NT, VT, WN> sm;
+    __shared__ StreamSmem<NT, VT, WN, MAT ? 1024 : 1> sm;
     __shared__ uint32_t shTicket;
     __shared__ uint64_t shPrefix;
     uint32_t s = blockIdx.x;
@@ -94,7 +94,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const uint64_t ib = dBeg - ia, ibE = dEnd - iaE;
 
     uint32_t* slot = MAT ? stage + uint64_t(s) * stage_cap : nullptr;
-    const uint64_t c = merge_segment_stream<NT, VT, WN, MAT, SEL>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm, slot);
+    const uint64_t c =
+        merge_segment_stream<NT, VT, WN, MAT, SEL, MAT ? 1024 : 1>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm, slot);
     const uint64_t total = block_sum_u64<NT>(c, sm.red);
 
     if (MAT) {

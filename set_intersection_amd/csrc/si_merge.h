@@ -320,9 +320,12 @@ struct StreamCfg {
     static_assert(VT <= 32, "match bits are kept in one 32-bit word");
 };
 
-template <int NT, int VT, int WN>
+// hits: a tile's matches compacted in order before one coalesced store (when
+// they fit; a materialising kernel only)
+template <int NT, int VT, int WN, int HB = 1>
 struct StreamSmem {
     alignas(16) uint32_t win[2 * StreamCfg<NT, VT, WN>::SPAN];  // A window [0, SPAN), B window [SPAN, 2 SPAN)
+    uint32_t hits[HB];
     uint32_t ends[NT];
     uint32_t wave_feas[NT / WAVE];
     uint32_t wave_tot[NT / WAVE];
@@ -385,7 +388,8 @@ struct StreamLoad {
         }
     }
 
-    __device__ __forceinline__ void commit(StreamSmem<NT, VT, WN>& sm) const {
+    template <typename Smem>
+    __device__ __forceinline__ void commit(Smem& sm) const {
 #pragma unroll
         for (int j = 0; j < C::CPT; ++j) {
             const uint32_t c = threadIdx.x + uint32_t(j) * NT;
@@ -425,10 +429,10 @@ __device__ __forceinline__ uint32_t split_search(const uint32_t* wa, uint32_t sa
 
 // SEL: one LDS read per merge step (the next element of the side just
 // consumed, selected by address) instead of re-reading both heads.
-template <int NT, int VT, int WN, bool MAT, bool SEL = false>
+template <int NT, int VT, int WN, bool MAT, bool SEL, int HB>
 __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
                                          uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
-                                         StreamSmem<NT, VT, WN>& sm, uint32_t* __restrict__ stage) {
+                                         StreamSmem<NT, VT, WN, HB>& sm, uint32_t* __restrict__ stage) {
     using C = StreamCfg<NT, VT, WN>;
     constexpr uint32_t T = C::T, W = C::W, M = C::M;
     const uint32_t tid = threadIdx.x;
@@ -602,16 +606,26 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
                 if (i < int(w)) before += t;
                 wtot += t;
             }
-            uint32_t* dst = stage + written + before + excl;
+            // compact in LDS, then one coalesced store; more than HB matches
+            // in a tile (dense intersections) go straight to the slot
+            const bool inLds = wtot <= uint32_t(HB);
             uint32_t mb = mbits;
+            uint32_t pos = before + excl;
             while (mb) {
                 const uint32_t hb = 31u - uint32_t(__builtin_clz(mb));  // earliest remaining match
                 const uint32_t j = uint32_t(__popc(tbits >> (hb + 1)));
-                *dst++ = rA[(sa + x0 + j) & M];
+                const uint32_t v = rA[(sa + x0 + j) & M];
+                if (inLds)
+                    sm.hits[pos] = v;
+                else
+                    stage[written + pos] = v;
+                ++pos;
                 mb &= ~(1u << hb);
             }
+            __syncthreads();  // matches read from the window before the refill overwrites their slots
+            if (inLds)
+                for (uint32_t i = tid; i < wtot; i += NT) stage[written + i] = sm.hits[i];
             written += wtot;
-            __syncthreads();  // staged values are read before the refill overwrites their slots
         }
 
         // 5. commit the refill
