@@ -93,7 +93,7 @@ def triangular_index(n, i, j):
 
 def all_pairs(coll, method="std", threads=None):
     """Triangular-order counts of all pairs; returns (counts, seconds)."""
-    threads = threads or os.cpu_count() or 1
+    threads = threads or min(16, os.cpu_count() or 1)
     k = coll.k
     counts = np.zeros(max(k * (k - 1) // 2, 1), dtype=np.uint32)
     offs = np.ascontiguousarray(coll.offsets, dtype=np.uint64)
@@ -103,7 +103,7 @@ def all_pairs(coll, method="std", threads=None):
 
 
 def pairs(coll, pa, pb, method="std", threads=None):
-    threads = threads or os.cpu_count() or 1
+    threads = threads or min(16, os.cpu_count() or 1)
     pa, pb = _arr(pa), _arr(pb)
     counts = np.zeros(max(pa.size, 1), dtype=np.uint32)
     offs = np.ascontiguousarray(coll.offsets, dtype=np.uint64)
@@ -114,7 +114,7 @@ def pairs(coll, pa, pb, method="std", threads=None):
 
 
 def edges(coll, u, v, threads=None):
-    threads = threads or os.cpu_count() or 1
+    threads = threads or min(16, os.cpu_count() or 1)
     u, v = _arr(u), _arr(v)
     counts = np.zeros(max(u.size, 1), dtype=np.uint32)
     offs = np.ascontiguousarray(coll.offsets, dtype=np.uint64)
@@ -124,7 +124,7 @@ def edges(coll, u, v, threads=None):
 
 
 def tile_matrix(coll, a0, a1, b0, b1, threads=None):
-    threads = threads or os.cpu_count() or 1
+    threads = threads or min(16, os.cpu_count() or 1)
     ld = b1 - b0
     C = np.zeros(max((a1 - a0) * ld, 1), dtype=np.uint32)
     offs = np.ascontiguousarray(coll.offsets, dtype=np.uint64)
