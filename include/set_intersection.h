@@ -201,9 +201,12 @@ int si_all_pairs_bitmap_async(si_dataset* ds, const si_params* params, si_buffer
  * S_{B.start+j}|, computed by an int8 MFMA GEMM of 0/1 incidence matrices
  * built on the device (int32 accumulation: exact below 2^31).
  * variant 0 = int8 MFMA / int32 accumulate, 1 = FP32-accumulate comparison
- * variant (exact only while every count is below 2^24), 2 = bitmap AND +
- * popcount.  ld = row stride of C in elements (>= B.len). */
-enum { SI_MM_I8 = 0, SI_MM_F32 = 1, SI_MM_POPCOUNT = 2 };
+ * variant (fp8 MFMA), 2 = bitmap AND + popcount, 3 = fp4 (e2m1) MFMA / FP32
+ * accumulate, twice the int8 rate.  Variants 1 and 3 are exact while every
+ * count is below 2^24 and fail with SI_ERR_RANGE for a universe above 2^24
+ * (the reference's cublasSgemm has the same bound).  ld = row stride of C in
+ * elements (>= B.len). */
+enum { SI_MM_I8 = 0, SI_MM_F32 = 1, SI_MM_POPCOUNT = 2, SI_MM_FP4 = 3 };
 int si_mm_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint32_t ld,
                  uint32_t* counts_host);
 int si_mm_counts_async(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint32_t ld,

@@ -12,13 +12,13 @@ from ctypes import byref, c_double, c_float, c_int32, c_uint32, c_uint64, c_void
 import numpy as np
 
 from . import _abi
-from ._abi import (MM_F32, MM_I8, MM_POPCOUNT, TECH_BITMAP_DYNAMIC, TECH_BITMAP_NAIVE, TECH_HI, TECH_IP,
+from ._abi import (MM_F32, MM_FP4, MM_I8, MM_POPCOUNT, TECH_BITMAP_DYNAMIC, TECH_BITMAP_NAIVE, TECH_HI, TECH_IP,
                    TECH_MM, TECH_OBS, TECHNIQUES, LibraryMissing, si_params, si_tile, si_tuning)
 from .datasets import Collection, read_bin, write_bin
 
 __all__ = ["Context", "Dataset", "Buffer", "SIError", "LibraryMissing", "Collection", "read_bin", "write_bin",
            "TECHNIQUES", "TECH_IP", "TECH_OBS", "TECH_HI", "TECH_BITMAP_NAIVE", "TECH_BITMAP_DYNAMIC",
-           "TECH_MM", "MM_I8", "MM_F32", "MM_POPCOUNT", "params", "tile", "run_slots"]
+           "TECH_MM", "MM_I8", "MM_F32", "MM_POPCOUNT", "MM_FP4", "params", "tile", "run_slots"]
 
 
 class SIError(RuntimeError):

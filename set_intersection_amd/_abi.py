@@ -20,7 +20,7 @@ STATUS_NAMES = {0: "SI_OK", -1: "SI_ERR_ARG", -2: "SI_ERR_HIP", -3: "SI_ERR_NOME
 TECH_IP, TECH_OBS, TECH_HI, TECH_BITMAP_NAIVE, TECH_BITMAP_DYNAMIC, TECH_MM = range(6)
 TECHNIQUES = {"ip": TECH_IP, "obs": TECH_OBS, "hi": TECH_HI, "bitmap_naive": TECH_BITMAP_NAIVE,
               "bitmap_dynamic": TECH_BITMAP_DYNAMIC, "mm": TECH_MM}
-MM_I8, MM_F32, MM_POPCOUNT = 0, 1, 2
+MM_I8, MM_F32, MM_POPCOUNT, MM_FP4 = 0, 1, 2, 3
 SI_TUNING_VERSION = 2
 
 

@@ -2,9 +2,12 @@
 // src/matrix_multiplication.cu executable (target name `mm`, CMakeLists.txt:152).
 // Same flags (--input, --output, --partition), tables, tile loop (:139-184) and
 // result file (writeResult<T, true>, io.hpp:104-136).  The incidence matrices
-// are built on the device and multiplied with int8 MFMA (int32 accumulation);
-// --variant f32 selects the FP32-accumulate comparison path (exact below 2^24),
-// --variant popcount the bitmap AND + popcount path.
+// are built on the device as bit rows and multiplied on the matrix cores:
+// --variant auto (default) takes the fp4 MFMA path (FP32 accumulation, exact
+// while counts stay below 2^24, as the reference's SGEMM) when the universe is
+// below 2^24 and the int8 path (int32 accumulation) otherwise; i8, fp4, f32
+// (fp8 MFMA, FP32 accumulation) and popcount (bitmap AND + popcount) select a
+// path explicitly.
 #include <algorithm>
 #include <cstdio>
 #include <string>
@@ -28,13 +31,13 @@ int main(int argc, char** argv) {
     if (si_open(0, &ctx) != SI_OK) return die(nullptr);
     si_device_info info;
     si_device_info_get(ctx, &info);
-    std::string input, output, variant = "i8";
+    std::string input, output, variant = "auto";
     uint32_t partition = 10000;
     si::Options options(argv[0], "Help");
     options.add("input", "Input dataset path", &input)
         .add("output", "Output result path", &output)
         .add("partition", "Number of sets to be processed per GPU invocation", &partition)
-        .add("variant", "i8 (int8 MFMA, default) | f32 (FP32 accumulate) | popcount", &variant)
+        .add("variant", "auto (default) | fp4 | i8 | f32 | popcount", &variant)
         .addFlag("help", "Print help");
     try {
         options.parse(argc, argv);
@@ -53,7 +56,8 @@ int main(int argc, char** argv) {
         si_close(ctx);
         return 1;
     }
-    int32_t v = variant == "f32" ? SI_MM_F32 : variant == "popcount" ? SI_MM_POPCOUNT : variant == "i8" ? SI_MM_I8 : -1;
+    int32_t v = variant == "f32" ? SI_MM_F32 : variant == "popcount" ? SI_MM_POPCOUNT : variant == "i8" ? SI_MM_I8
+              : variant == "fp4" ? SI_MM_FP4 : variant == "auto" ? SI_MM_FP4 : -1;
     if (v < 0 || partition == 0) {
         std::printf("bad --variant or --partition\n");
         si_close(ctx);
@@ -76,6 +80,7 @@ int main(int argc, char** argv) {
     std::vector<si::tile_pair> runs = si::findTilePairs(d.cardinality, partition);
     const uint64_t combinations = uint64_t(partition) * partition;
     const uint64_t K = uint64_t(d.universe) + 1;
+    if (variant == "auto" && K > (uint64_t(1) << 24)) v = SI_MM_I8;  // FP32 counts would round
     const uint64_t deviceMemory = (K * partition * 2) + combinations * 4;
     si::printTable("Launch info", {{"Partition", si::toText(partition)},
                                    {"GPU invocations", si::toText(runs.size())},

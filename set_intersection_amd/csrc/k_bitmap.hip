@@ -161,6 +161,35 @@ __global__ void k_bm_build_rows(const uint32_t* __restrict__ base, const uint64_
     }
 }
 
+// The same rows built in LDS when a row fits (w64 even, at most
+// LDS_ROW_BYTES): a workgroup clears the row in LDS, ORs its set's bits in
+// (plain stores inside a wave's chunk, atomics at chunk edges) and writes the
+// whole row, zeros included, with 16-byte stores, so rows need no clearing.
+constexpr uint64_t LDS_ROW_BYTES = 64 * 1024;
+constexpr int ROWS_NT = 256;
+
+__global__ __launch_bounds__(ROWS_NT) void k_bm_build_rows_lds(const uint32_t* __restrict__ base,
+                                                               const uint64_t* __restrict__ offsets, uint32_t s0,
+                                                               uint32_t ns, uint64_t w64,
+                                                               unsigned long long* __restrict__ rows) {
+    extern __shared__ uint4 lrow4[];
+    uint32_t* lrow = reinterpret_cast<uint32_t*>(lrow4);
+    constexpr uint32_t nw = ROWS_NT / WAVE;
+    const uint32_t w = threadIdx.x / WAVE;
+    const uint32_t q4 = uint32_t(w64 / 2);  // 16-byte pieces of a row
+    for (uint32_t s = blockIdx.x; s < ns; s += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < q4; i += ROWS_NT) lrow4[i] = uint4{0u, 0u, 0u, 0u};
+        __syncthreads();
+        const uint64_t o = offsets[s0 + s], n = offsets[s0 + s + 1] - o;
+        const uint64_t chunks = (n + WAVE - 1) / WAVE;
+        for (uint64_t c = w; c < chunks; c += uint64_t(nw) * BUILD_UNROLL) or_bits_chunks(base + o, n, c, nw, lrow);
+        __syncthreads();
+        uint4* dst = reinterpret_cast<uint4*>(rows + uint64_t(s) * w64);
+        for (uint32_t i = threadIdx.x; i < q4; i += ROWS_NT) dst[i] = lrow4[i];
+        __syncthreads();  // the row is read out before the next set clears it
+    }
+}
+
 // AND + popcount over bit rows: C(i, j) = popcount(R_i & R_j) for i in row
 // block [i0, i0+ni) of RA and j in [j0, j0+nj) of RB.
 //   TRI: write counts[tri_index(n, i, j)] for j > i only (RA == RB).
@@ -302,11 +331,23 @@ int bitmap_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_
     return SI_OK;
 }
 
-// Bit rows of sets [s0, s0+ns) into `rows` (ns x w64 words), zeroed first.
+// Bit rows of sets [s0, s0+ns) into `rows` (ns x w64 words): every word is
+// written when a row fits in LDS; otherwise the rows are zeroed first (by
+// the caller when zero is false) and the set bits ORed in.
+bool bit_rows_in_lds(uint64_t w64) { return w64 % 2 == 0 && w64 * 8 <= LDS_ROW_BYTES; }
+
 int build_bit_rows(si_dataset* ds, uint32_t s0, uint32_t ns, uint64_t w64, unsigned long long* rows, bool zero) {
     si_context* ctx = ds->ctx;
-    if (zero) SI_HIP(hipMemsetAsync(rows, 0, uint64_t(ns) * w64 * 8, ctx->stream));
     if (ns == 0) return SI_OK;
+    if (bit_rows_in_lds(w64)) {
+        KernelScope ks(ctx, "bitmap_build_rows");
+        const uint32_t grid = std::min<uint32_t>(ns, uint32_t(ctx->num_cus) * 8);
+        hipLaunchKernelGGL(k_bm_build_rows_lds, dim3(grid), dim3(ROWS_NT), size_t(w64 * 8), ctx->stream, ds->d_elems,
+                           ds->d_offsets, s0, ns, w64, rows);
+        SI_HIP(hipGetLastError());
+        return SI_OK;
+    }
+    if (zero) SI_HIP(hipMemsetAsync(rows, 0, uint64_t(ns) * w64 * 8, ctx->stream));
     KernelScope ks(ctx, "bitmap_build_rows");
     const uint32_t grid = std::min<uint32_t>(ns, uint32_t(ctx->num_cus) * 16);
     hipLaunchKernelGGL(k_bm_build_rows, dim3(grid), dim3(NT), 0, ctx->stream, ds->d_elems, ds->d_offsets, s0, ns, w64,

@@ -33,6 +33,15 @@
 //  * The FP32-accumulate comparison variant expands the same bits to the fp8
 //    (e4m3) encoding of 1.0 (0x38) and uses v_mfma_f32_16x16x32_fp8_fp8:
 //    exact only while every count is below 2^24.
+//  * The fp4 variant feeds v_mfma_scale_f32_16x16x128_f8f6f4 with e2m1
+//    nibbles (scales 2^0): 128 universe values per MFMA, twice the K of the
+//    int8 instruction in the same issue slots.  A lane's 32 nibbles come from
+//    one 32-bit word with one or two operations per dword: nibble i of dword
+//    j holds bit 4i + j, masked in place where the e2m1 code of the single set
+//    bit is a power of two (bit 0: 0.5, bit 1: 1.0, bit 2: 2.0; bit 3 is the
+//    sign, so it is shifted to bit 1), and B takes the reciprocal weights
+//    (2.0, 1.0, 0.5, 1.0), so every product of two set bits is exactly 1.0.
+//    fp32 accumulation: exact while every count is below 2^24.
 #include <type_traits>
 
 #include "si_device.h"
@@ -54,6 +63,20 @@ constexpr int RS = 2 * KC + 1;     // LDS row stride in dwords (odd)
 // as 0/1 (int8) or 0/0x38 (fp8 1.0).  Any fixed assignment of universe
 // values to MFMA K positions is correct as long as A and B use the same one;
 // this one costs a shift and a mask per dword.
+// fp4 operands (see above): A's dword j carries weight 2^(j-1) (j < 3) or 1,
+// B's the reciprocal
+__device__ __forceinline__ v4i expand_a4(uint32_t x) {
+    return v4i{int(x & 0x11111111u), int(x & 0x22222222u), int(x & 0x44444444u), int((x >> 2) & 0x22222222u)};
+}
+__device__ __forceinline__ v4i expand_b4(uint32_t x) {
+    const uint32_t y = x >> 2;
+    return v4i{int((x << 2) & 0x44444444u), int(x & 0x22222222u), int(y & 0x11111111u), int(y & 0x22222222u)};
+}
+typedef int v8i __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ v8i f4_operand(const v4i& f) { return v8i{f[0], f[1], f[2], f[3], 0, 0, 0, 0}; }
+
+enum { MODE_I8 = 0, MODE_F8 = 1, MODE_F4 = 2 };
+
 template <bool F8>
 __device__ __forceinline__ v4i expand32(uint32_t x, uint32_t p) {
     const uint32_t y = x >> (4 * p);
@@ -91,7 +114,7 @@ __device__ __forceinline__ void mm_tile_of(const MmGrid& g, uint32_t t, uint32_t
 // columns [64w, 64w + 64) (8 x 4 MFMA blocks), so an expanded fragment feeds
 // 4 (A) or 8 (B) MFMAs.  self: RA == RB, tiles on or above the diagonal only,
 // each stored to C and Cᵀ.
-template <bool F32>
+template <int MODE>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mm_bits(const uint64_t* __restrict__ RA, uint32_t ni,
                                                  const uint64_t* __restrict__ RB, uint32_t nj, uint64_t w64,
                                                  MmGrid g, uint32_t* __restrict__ C, uint64_t ld) {
@@ -142,7 +165,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
 
-    using Acc = typename std::conditional<F32, v4f, v4i>::type;
+    using Acc = typename std::conditional<MODE != MODE_I8, v4f, v4i>::type;
     Acc acc[8][4];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
@@ -161,7 +184,54 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (c + 1 < c1) load(uint64_t(c + 1) * KC);
         const uint32_t* dA = sA[buf];
         const uint32_t* dB = sB[buf];
-        if constexpr (!F32) {
+        if constexpr (MODE == MODE_F4) {
+            // as the int8 pipeline below, with K steps of 128 universe values
+            // (4 dwords of a row; lane quarter q takes dword q)
+            const uint32_t* qA = dA + (r16 * RS + q);
+            const uint32_t* qB = dB + ((wn * 64 + r16) * RS + q);
+            uint32_t wa[8], wb[4];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) wa[i] = qA[i * 16 * RS];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) wb[j] = qB[j * 16 * RS];
+            v4i fb[4], fbn[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) fb[j] = expand_b4(wb[j]);
+            v4i fan = expand_a4(wa[0]);
+#pragma unroll 2
+            for (int kk = 0; kk < KC / 2; ++kk) {
+                const bool more = kk + 1 < KC / 2;
+                const int k4 = more ? 4 * (kk + 1) : 4 * kk;
+                uint32_t na[8], nb[4];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) na[i] = qA[i * 16 * RS + k4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) nb[j] = qB[j * 16 * RS + k4];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const v4i fa = fan;
+                    if (i < 7) fan = expand_a4(wa[i + 1]);
+                    else fan = expand_a4(na[0]);
+                    if (i >= 3 && i < 7) fbn[i - 3] = expand_b4(nb[i - 3]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                            f4_operand(fa), f4_operand(fb[j]), acc[i][j], 4, 4, 0, 127, 0, 127);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                        if (i >= 3 && i < 7)
+                            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+                        else
+                            __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) wa[i] = na[i];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fb[j] = fbn[j];
+            }
+        } else if constexpr (MODE == MODE_I8) {
             // software pipeline over the 8 A fragments of a K step: while the
             // 4 MFMAs of fragment i issue, fragment i+1 (and from i = 3 on one
             // B fragment of the next K step) is expanded, so the expansion
@@ -281,6 +351,9 @@ int mm_tile_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint64
     if (ni == 0 || nj == 0) return SI_OK;
     const bool self = (A.start == B.start && A.end == B.end);
     const uint64_t K = uint64_t(std::max(ds->max_elem, ds->universe)) + 1;
+    // fp32 accumulation is exact only while every count (at most K) is below 2^24
+    if ((variant == SI_MM_F32 || variant == SI_MM_FP4) && K > (uint64_t(1) << 24))
+        return fail(SI_ERR_RANGE, "universe above 2^24: FP32-accumulated counts could round (use SI_MM_I8)");
     if (variant == SI_MM_POPCOUNT) {
         const uint64_t w64 = (K + 63) / 64;
         const uint64_t rows = self ? ni : uint64_t(ni) + nj;
@@ -298,7 +371,13 @@ int mm_tile_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint64
     SI_TRY(ctx->ws_aux.ensure(rows * w64 * 8));
     auto* RA = ctx->ws_aux.as<unsigned long long>();
     auto* RB = self ? RA : RA + rA * w64;
-    SI_HIP(hipMemsetAsync(RA, 0, rows * w64 * 8, ctx->stream));
+    if (bit_rows_in_lds(w64)) {
+        // the builder writes whole rows: clear only the padding rows
+        if (rA > ni) SI_HIP(hipMemsetAsync(RA + uint64_t(ni) * w64, 0, (rA - ni) * w64 * 8, ctx->stream));
+        if (!self && rB > nj) SI_HIP(hipMemsetAsync(RB + uint64_t(nj) * w64, 0, (rB - nj) * w64 * 8, ctx->stream));
+    } else {
+        SI_HIP(hipMemsetAsync(RA, 0, rows * w64 * 8, ctx->stream));
+    }
     SI_TRY(build_bit_rows(ds, A.start, ni, w64, RA, false));
     if (!self) SI_TRY(build_bit_rows(ds, B.start, nj, w64, RB, false));
     const uint32_t tM = uint32_t(rA / TILE), tN = uint32_t(rB / TILE);
@@ -323,11 +402,13 @@ int mm_tile_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint64
     const uint32_t units = g.nfull + g.ntail * g.nsplit;
     const uint64_t* bA = reinterpret_cast<const uint64_t*>(RA);
     const uint64_t* bB = reinterpret_cast<const uint64_t*>(RB);
-    KernelScope ks(ctx, variant == SI_MM_F32 ? "mm_gemm_f32" : "mm_gemm_i8");
+    KernelScope ks(ctx, variant == SI_MM_F32 ? "mm_gemm_f32" : variant == SI_MM_FP4 ? "mm_gemm_fp4" : "mm_gemm_i8");
     if (variant == SI_MM_F32)
-        hipLaunchKernelGGL((k_mm_bits<true>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
+        hipLaunchKernelGGL((k_mm_bits<MODE_F8>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
+    else if (variant == SI_MM_FP4)
+        hipLaunchKernelGGL((k_mm_bits<MODE_F4>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
     else
-        hipLaunchKernelGGL((k_mm_bits<false>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
+        hipLaunchKernelGGL((k_mm_bits<MODE_I8>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
     SI_HIP(hipGetLastError());
     return SI_OK;
 }

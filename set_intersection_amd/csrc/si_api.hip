@@ -171,7 +171,7 @@ int si_mm_counts_async(si_dataset* ds, si_tile A, si_tile B, int32_t variant, ui
     if (!ds || !counts) return fail(SI_ERR_ARG, "null argument");
     SI_TRY(check_tile(ds, A));
     SI_TRY(check_tile(ds, B));
-    if (variant < SI_MM_I8 || variant > SI_MM_POPCOUNT) return fail(SI_ERR_ARG, "unknown mm variant");
+    if (variant < SI_MM_I8 || variant > SI_MM_FP4) return fail(SI_ERR_ARG, "unknown mm variant");
     const uint64_t ni = A.end - A.start, nj = B.end - B.start;
     if (ld < nj) return fail(SI_ERR_ARG, "ld smaller than B.len");
     const uint64_t need = ni ? ((ni - 1) * ld + nj) : 0;

@@ -137,6 +137,7 @@ int hi_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n,
 // bitmaps (k_bitmap.hip)
 int bitmap_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);
 int bitmap_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
+bool bit_rows_in_lds(uint64_t w64);
 int build_bit_rows(si_dataset* ds, uint32_t s0, uint32_t ns, uint64_t w64, unsigned long long* rows, bool zero = true);
 int popcount_pairs(si_context* ctx, const unsigned long long* RA, uint32_t ni, const unsigned long long* RB, uint32_t nj,
                    uint64_t w64, bool tri, uint32_t* out, uint64_t ld);

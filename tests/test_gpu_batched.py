@@ -71,7 +71,7 @@ def test_batched_autotune(ctx):
 
 
 # ------------------------------------------------------------ bitmap GEMM --
-@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32, si.MM_POPCOUNT])
+@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32, si.MM_POPCOUNT, si.MM_FP4])
 def test_mm_tiles(ctx, variant):
     coll = datasets.density_collection(300, 1 << 12, 0.01, 0.3, 11)
     ds = ctx.upload(coll)
@@ -84,6 +84,19 @@ def test_mm_tiles(ctx, variant):
         assert np.array_equal(got, exp), ((a0, a1), (b0, b1))
 
 
+def test_mm_fp32_range(ctx):
+    """The FP32-accumulated variants refuse a universe above 2^24 (counts could
+    round); the int8 path takes it."""
+    coll = datasets.density_collection(4, 1 << 12, 0.1, 0.2, 5)
+    coll.universe = (1 << 24) + 5
+    ds = ctx.upload(coll)
+    A = si.tile(0, 0, 4)
+    for v in (si.MM_FP4, si.MM_F32):
+        with pytest.raises(si.SIError):
+            ds.mm_counts(A, A, v)
+    assert np.array_equal(ds.mm_counts(A, A, si.MM_I8), oracle.tile_matrix(coll, 0, 4, 0, 4))
+
+
 def test_mm_ld_padding(ctx):
     coll = datasets.density_collection(70, 1 << 10, 0.05, 0.5, 3)
     ds = ctx.upload(coll)
@@ -93,7 +106,7 @@ def test_mm_ld_padding(ctx):
     assert np.array_equal(got[:, :30], exp)
 
 
-@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32])
+@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32, si.MM_FP4])
 def test_mm_c4_shape_slice(ctx, variant):
     """A 1024-set slice of the 8192 x 2^16 density workload (full K)."""
     coll = datasets.density_collection(1024, 1 << 16, 0.01, 0.10, 20261203)
@@ -104,7 +117,7 @@ def test_mm_c4_shape_slice(ctx, variant):
     assert np.array_equal(got, exp)
 
 
-@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32])
+@pytest.mark.parametrize("variant", [si.MM_I8, si.MM_F32, si.MM_FP4])
 def test_mm_split_tail(ctx, variant):
     """6000 sets: 1128 self tiles, more than one round of resident workgroups,
     so the last partial round is split along K and accumulated atomically.
