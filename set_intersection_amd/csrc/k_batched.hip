@@ -405,9 +405,63 @@ __device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uin
     }
 }
 
+// A wave's share [ws, we) of an item's concatenated partner lists, partner
+// by partner: the 64 lanes stream a partner's elements (U loads in flight per
+// lane) and probe them, and the partner's count is a sum of ballot
+// popcounts.  A partner cut by ws or we leaves its partial count in cutSet /
+// cutCnt (slot 0: the one that begins before ws, which is also the one when
+// the whole share lies inside a single partner; slot 1: the one that
+// continues past we).  Partners of big rows are long (hundreds of elements
+// on average), so whole-wave runs waste few lanes and need no
+// position-to-partner search.
+template <int U, typename ProbeFn>
+__device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32_t j0, uint32_t j1, uint32_t ws,
+                                          uint32_t we, const uint32_t* __restrict__ elems, ProbeFn probeFn,
+                                          uint32_t* __restrict__ out, uint32_t* cutSet, uint32_t* cutCnt) {
+    const uint32_t lane = lane_id();
+    // first partner whose elements end past ws
+    uint32_t lo = j0, hi = j1 - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (part[mid].z > ws) hi = mid; else lo = mid + 1;
+    }
+    uint32_t start = (lo == j0) ? 0u : part[lo - 1].z;
+    for (uint32_t q = lo; q < j1 && start < we; ++q) {
+        const uint4 P = part[q];
+        const uint32_t end = P.z;
+        const uint32_t a = start > ws ? start : ws, b = end < we ? end : we;
+        const uint32_t* src = elems + (((uint64_t(P.y) << 32) | P.x) + (a - start));
+        const uint32_t n = b - a;
+        uint32_t cnt = 0;
+        for (uint32_t i = 0; i < n; i += U * WAVE) {
+            uint32_t x[U];
+            bool v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = i + uint32_t(u) * WAVE + lane;
+                v[u] = k < n;
+                x[u] = v[u] ? src[k] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) cnt += uint32_t(__popcll(__ballot(v[u] && probeFn(x[u]))));
+        }
+        if (lane == 0) {
+            if (a == start && b == end) {
+                out[P.w] = cnt;
+            } else {
+                const int slot = a != start ? 0 : 1;
+                cutSet[slot] = q;
+                cutCnt[slot] = cnt;
+            }
+        }
+        start = end;
+    }
+}
+
 // Rows whose structure is shared by a workgroup (BITMAP, HASH, SEARCH):
 // persistent workgroups pop items; the waves split the item's partner
-// elements evenly; partners cut by wave boundaries are completed through LDS.
+// elements evenly and stream them partner by partner; partners cut by wave
+// boundaries are completed through LDS.
 template <int NT, int LDSW>
 __global__ __launch_bounds__(NT) void k_edges_wg(const Item* __restrict__ items, uint32_t nitems,
                                                  uint32_t* __restrict__ queue, const uint32_t* __restrict__ elems,
@@ -443,8 +497,8 @@ __global__ __launch_bounds__(NT) void k_edges_wg(const Item* __restrict__ items,
         const uint32_t T = part[I.j1 - 1].z;
         const uint32_t ws = uint32_t((uint64_t(T) * w) / NW), we = uint32_t((uint64_t(T) * (w + 1)) / NW);
         if (ws < we)
-            edge_batches(part, I.j0, I.j1, ws, we, elems,
-                         [&](uint32_t x) { return probe<LDSW>(sm, mode, x, S, rowN, logTS, hasMax); }, out,
+            edge_runs<4>(part, I.j0, I.j1, ws, we, elems,
+                         [&](uint32_t x) { return probe<LDSW>(sm, mode, x, S, rowN, logTS, hasMax) != 0u; }, out,
                          &sm.pset[2 * w], &sm.pcnt[2 * w]);
         __syncthreads();
         if (tid == 0) {
