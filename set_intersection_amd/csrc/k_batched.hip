@@ -406,14 +406,15 @@ __device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uin
 }
 
 // A wave's share [ws, we) of an item's concatenated partner lists, partner
-// by partner: the 64 lanes stream a partner's elements (U loads in flight per
-// lane) and probe them, and the partner's count is a sum of ballot
-// popcounts.  A partner cut by ws or we leaves its partial count in cutSet /
-// cutCnt (slot 0: the one that begins before ws, which is also the one when
-// the whole share lies inside a single partner; slot 1: the one that
-// continues past we).  Partners of big rows are long (hundreds of elements
-// on average), so whole-wave runs waste few lanes and need no
-// position-to-partner search.
+// by partner: the records of 64 partners at a time are loaded one per lane
+// (no dependent record load per partner), then the 64 lanes stream each
+// partner's elements (U loads in flight per lane) and probe them; a
+// partner's count is a sum of ballot popcounts.  A partner cut by ws or we
+// leaves its partial count in cutSet / cutCnt (slot 0: the one that begins
+// before ws, which is also the one when the whole share lies inside a single
+// partner; slot 1: the one that continues past we).  Partners of big rows
+// are long (hundreds of elements on average), so whole-wave runs waste few
+// lanes and need no position-to-partner search.
 template <int U, typename ProbeFn>
 __device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32_t j0, uint32_t j1, uint32_t ws,
                                           uint32_t we, const uint32_t* __restrict__ elems, ProbeFn probeFn,
@@ -425,36 +426,47 @@ __device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32
         const uint32_t mid = (lo + hi) >> 1;
         if (part[mid].z > ws) hi = mid; else lo = mid + 1;
     }
-    uint32_t start = (lo == j0) ? 0u : part[lo - 1].z;
-    for (uint32_t q = lo; q < j1 && start < we; ++q) {
-        const uint4 P = part[q];
-        const uint32_t end = P.z;
-        const uint32_t a = start > ws ? start : ws, b = end < we ? end : we;
-        const uint32_t* src = elems + (((uint64_t(P.y) << 32) | P.x) + (a - start));
-        const uint32_t n = b - a;
-        uint32_t cnt = 0;
-        for (uint32_t i = 0; i < n; i += U * WAVE) {
-            uint32_t x[U];
-            bool v[U];
+    uint32_t qb = lo;
+    uint32_t prevEnd = (qb == j0) ? 0u : part[qb - 1].z;
+    while (qb < j1 && prevEnd < we) {
+        const bool pv = qb + lane < j1;
+        const uint4 R = pv ? part[qb + lane] : make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+        const uint32_t nrec = j1 - qb < uint32_t(WAVE) ? j1 - qb : uint32_t(WAVE);
+        uint32_t start = prevEnd;
+        for (uint32_t r = 0; r < nrec && start < we; ++r) {
+            const uint32_t end = uint32_t(__builtin_amdgcn_readlane(int(R.z), int(r)));
+            const uint32_t offLo = uint32_t(__builtin_amdgcn_readlane(int(R.x), int(r)));
+            const uint32_t offHi = uint32_t(__builtin_amdgcn_readlane(int(R.y), int(r)));
+            const uint32_t a = start > ws ? start : ws, b = end < we ? end : we;
+            const uint32_t* src = elems + (((uint64_t(offHi) << 32) | offLo) + (a - start));
+            const uint32_t n = b - a;
+            uint32_t cnt = 0;
+            for (uint32_t i = 0; i < n; i += U * WAVE) {
+                // straight-line loads: positions past the partner's end re-read its last element
+                uint32_t x[U];
+                bool v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t k = i + uint32_t(u) * WAVE + lane;
-                v[u] = k < n;
-                x[u] = v[u] ? src[k] : 0u;
-            }
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t k = i + uint32_t(u) * WAVE + lane;
+                    v[u] = k < n;
+                    x[u] = src[v[u] ? k : n - 1];
+                }
 #pragma unroll
-            for (int u = 0; u < U; ++u) cnt += uint32_t(__popcll(__ballot(v[u] && probeFn(x[u]))));
-        }
-        if (lane == 0) {
-            if (a == start && b == end) {
-                out[P.w] = cnt;
-            } else {
-                const int slot = a != start ? 0 : 1;
-                cutSet[slot] = q;
-                cutCnt[slot] = cnt;
+                for (int u = 0; u < U; ++u) cnt += uint32_t(__popcll(__ballot(probeFn(x[u]) & v[u])));
             }
+            if (lane == 0) {
+                if (a == start && b == end) {
+                    out[uint32_t(__builtin_amdgcn_readlane(int(R.w), int(r)))] = cnt;
+                } else {
+                    const int slot = a != start ? 0 : 1;
+                    cutSet[slot] = qb + r;
+                    cutCnt[slot] = cnt;
+                }
+            }
+            start = end;
         }
-        start = end;
+        prevEnd = uint32_t(__builtin_amdgcn_readlane(int(R.z), int(nrec - 1)));
+        qb += nrec;
     }
 }
 
@@ -496,10 +508,16 @@ __global__ __launch_bounds__(NT) void k_edges_wg(const Item* __restrict__ items,
         const uint32_t* S = elems + rowOff;
         const uint32_t T = part[I.j1 - 1].z;
         const uint32_t ws = uint32_t((uint64_t(T) * w) / NW), we = uint32_t((uint64_t(T) * (w + 1)) / NW);
-        if (ws < we)
-            edge_runs<4>(part, I.j0, I.j1, ws, we, elems,
-                         [&](uint32_t x) { return probe<LDSW>(sm, mode, x, S, rowN, logTS, hasMax) != 0u; }, out,
-                         &sm.pset[2 * w], &sm.pcnt[2 * w]);
+        if (ws < we) {
+            if (mode == MODE_BITMAP)  // the common case, without the generic probe's branches
+                edge_runs<4>(part, I.j0, I.j1, ws, we, elems,
+                             [&](uint32_t x) { return ((sm.tab[x >> 5] >> (x & 31)) & 1u) != 0u; }, out,
+                             &sm.pset[2 * w], &sm.pcnt[2 * w]);
+            else
+                edge_runs<4>(part, I.j0, I.j1, ws, we, elems,
+                             [&](uint32_t x) { return probe<LDSW>(sm, mode, x, S, rowN, logTS, hasMax) != 0u; },
+                             out, &sm.pset[2 * w], &sm.pcnt[2 * w]);
+        }
         __syncthreads();
         if (tid == 0) {
             for (int s = 0; s < 2 * NW; ++s) {
