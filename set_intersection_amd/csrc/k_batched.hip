@@ -199,12 +199,27 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
             for (uint64_t c0 = ws; c0 < we; c0 += WAVE * VT) {
                 const uint64_t c1 = (c0 + WAVE * VT < we) ? c0 + WAVE * VT : we;
                 uint64_t M[VT];
+                if (mode == MODE_BITMAP) {
+                    // the common case, branch-free: positions past c1 re-read c1 - 1
+                    uint32_t x[VT];
 #pragma unroll
-                for (int v = 0; v < VT; ++v) {
-                    const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
-                    uint32_t hit = 0;
-                    if (p < c1) hit = probe<LDSW>(sm, mode, selem[p], S, rowN, logTS, hasMax);
-                    M[v] = __ballot(hit);
+                    for (int v = 0; v < VT; ++v) {
+                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
+                        x[v] = selem[p < c1 ? p : c1 - 1];
+                    }
+#pragma unroll
+                    for (int v = 0; v < VT; ++v) {
+                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
+                        M[v] = __ballot(((sm.tab[x[v] >> 5] >> (x[v] & 31)) & 1u) && p < c1);
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VT; ++v) {
+                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
+                        uint32_t hit = 0;
+                        if (p < c1) hit = probe<LDSW>(sm, mode, selem[p], S, rowN, logTS, hasMax);
+                        M[v] = __ballot(hit);
+                    }
                 }
                 // sets overlapping [c0, c1): jc, jc+1, ...; lane t takes jc + g*64 + t
                 uint32_t nextJc = jc, nextCarry = 0;
