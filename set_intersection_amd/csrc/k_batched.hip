@@ -212,6 +212,18 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                         const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
                         M[v] = __ballot(((sm.tab[x[v] >> 5] >> (x[v] & 31)) & 1u) && p < c1);
                     }
+                } else if (mode == MODE_HASH) {
+                    uint32_t x[VT];
+#pragma unroll
+                    for (int v = 0; v < VT; ++v) {
+                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
+                        x[v] = selem[p < c1 ? p : c1 - 1];
+                    }
+#pragma unroll
+                    for (int v = 0; v < VT; ++v) {
+                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
+                        M[v] = __ballot(probe<LDSW>(sm, MODE_HASH, x[v], S, rowN, logTS, hasMax) && p < c1);
+                    }
                 } else {
 #pragma unroll
                     for (int v = 0; v < VT; ++v) {
