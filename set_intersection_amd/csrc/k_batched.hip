@@ -70,7 +70,7 @@ __device__ __forceinline__ uint64_t bit_range(int lo, int hi) {
 
 template <int LDSW>
 struct RowSmem {
-    uint32_t tab[LDSW];
+    alignas(16) uint32_t tab[LDSW];
     uint32_t item, row, mode, hasMax, logTS;
     uint32_t pset[2 * 32], pcnt[2 * 32];
 };
@@ -79,12 +79,17 @@ template <int NT, int LDSW>
 __device__ void build_row(RowSmem<LDSW>& sm, const uint32_t* __restrict__ S, uint64_t n, uint32_t mode,
                           uint32_t bmWords) {
     if (mode == MODE_BITMAP) {
-        for (uint32_t i = threadIdx.x; i < bmWords; i += NT) sm.tab[i] = 0;
+        // 16-byte clears, then the sorted row's bits by wave chunks with eight
+        // chunk loads in flight (plain stores inside a chunk, atomics at its
+        // edges)
+        for (uint32_t i = threadIdx.x; i < bmWords / 4; i += NT)
+            reinterpret_cast<uint4*>(sm.tab)[i] = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t i = (bmWords & ~3u) + threadIdx.x; i < bmWords; i += NT) sm.tab[i] = 0;
         __syncthreads();
-        for (uint64_t i = threadIdx.x; i < n; i += NT) {
-            const uint32_t x = S[i];
-            atomicOr(&sm.tab[x >> 5], 1u << (x & 31));
-        }
+        constexpr uint32_t NW = NT / WAVE;
+        const uint64_t chunks = (n + WAVE - 1) / WAVE;
+        for (uint64_t c = threadIdx.x / WAVE; c < chunks; c += uint64_t(NW) * BUILD_UNROLL)
+            or_bits_chunks(S, n, c, NW, sm.tab);
     } else if (mode == MODE_HASH) {
         uint32_t logTS = 6;
         while ((1ull << logTS) < 2 * n) ++logTS;

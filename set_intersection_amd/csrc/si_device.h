@@ -177,4 +177,53 @@ __device__ __forceinline__ void tri_decode(uint64_t p, uint64_t L, uint32_t& i, 
     j = uint32_t(p - rowStart(ii) + uint64_t(ii) + 1);
 }
 
+// Sets bit x of the 32-bit-word bitmap bm for the elements x of the sorted
+// unique list S[c*64, c*64+64) handled by this wave (chunk c; x is this lane's
+// element, valid when c*64 + lane < n).  Elements of one word are
+// consecutive, so lanes that share a word OR their bits together (segmented
+// OR over the wave with shuffles) and the segment's first lane writes the
+// word: a plain store inside the chunk, an atomic OR only for the chunk's
+// first and last word, which the neighbouring chunks may share.  bm must be
+// zero beforehand.
+__device__ __forceinline__ void or_bits_chunk(uint32_t x, bool valid, uint32_t* __restrict__ bm) {
+    const uint32_t lane = lane_id();
+    const uint32_t w = valid ? (x >> 5) : 0xFFFFFFFFu;
+    uint32_t bits = valid ? (1u << (x & 31)) : 0u;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t vb = __shfl_down(bits, off);
+        const uint32_t vw = __shfl_down(w, off);
+        if (lane + off < WAVE && vw == w) bits |= vb;
+    }
+    const uint32_t wprev = __shfl_up(w, 1);
+    const uint32_t w0 = __shfl(w, 0), wlast = __shfl(w, WAVE - 1);
+    if (valid && (lane == 0 || wprev != w)) {
+        if (w == w0 || w == wlast)
+            atomicOr(&bm[w], bits);
+        else
+            bm[w] = bits;
+    }
+}
+
+// Chunks cb, cb + stride, ... (UNROLL of them) of S[0, n): all loads are
+// issued before the first chunk is combined, so a wave keeps UNROLL chunk
+// loads in flight.
+constexpr int BUILD_UNROLL = 8;
+
+__device__ __forceinline__ void or_bits_chunks(const uint32_t* __restrict__ S, uint64_t n, uint64_t cb,
+                                               uint64_t stride, uint32_t* __restrict__ bm) {
+    const uint32_t lane = lane_id();
+    uint32_t x[BUILD_UNROLL];
+    bool v[BUILD_UNROLL];
+#pragma unroll
+    for (int u = 0; u < BUILD_UNROLL; ++u) {
+        const uint64_t i = (cb + uint64_t(u) * stride) * WAVE + lane;
+        v[u] = i < n;
+        x[u] = v[u] ? S[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < BUILD_UNROLL; ++u)
+        if (cb + uint64_t(u) * stride < (n + WAVE - 1) / WAVE) or_bits_chunk(x[u], v[u], bm);
+}
+
 }  // namespace si_dev
