@@ -15,8 +15,20 @@
 // MI355X design:
 //  * Bitmaps have universe+1 bits (elements reach `universe`; the reference
 //    sizes them one bit short, bitmap_naive.cu:68).
-//  * bitmap_dynamic SISI builds the SMALLER set (fewer atomics) and probes the
-//    larger; counts fold through per-workgroup partials (no reduce kernel).
+//  * bitmap_dynamic SISI is tiled by value: the universe is cut into buckets
+//    of 2^bits values (bits chosen so that a bucket holds about 4096
+//    elements of the larger set, at most 2^18 values = 32 KiB), and a
+//    workgroup takes a run of consecutive buckets; per bucket it clears an
+//    LDS bitmap, ORs in the bucket's elements of the SMALLER set (LDS
+//    atomics) and probes the bucket's elements of the larger set against it;
+//    a thread holds up to 16 elements of each side in registers, and the
+//    next bucket's are loaded while the current one is processed.
+//    The reference's single global bitmap of the whole universe (512 MiB at
+//    2^32, bitmap_dynamic.cu:240-276) becomes a few KiB of LDS; both sets
+//    stream once from HBM in order, with no dependent reads.  Bucket bounds
+//    in each set are binary searches at the workgroup's start; buckets
+//    without elements on either side are skipped.  Counts fold through
+//    per-workgroup partials (no reduce kernel).
 //  * bitmap_dynamic runs keep the row bitmap in LDS when universe+1 <= 2^20
 //    bits (128 KiB of the 160 KiB LDS), else in a per-workgroup global region;
 //    each pair's hits are reduced with wave shuffles (one atomic per wave,
@@ -33,26 +45,88 @@ namespace {
 using namespace si_dev;
 
 // ------------------------------------------------------- bitmap_dynamic --
-__global__ void k_bm_build(const uint32_t* __restrict__ S, uint64_t n, uint32_t* __restrict__ bm) {
-    const uint64_t chunks = (n + WAVE - 1) / WAVE;
-    const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / WAVE);
-    for (uint64_t c = uint64_t(blockIdx.x) * (blockDim.x / WAVE) + threadIdx.x / WAVE; c < chunks;
-         c += waves * BUILD_UNROLL)
-        or_bits_chunks(S, n, c, waves, bm);
-}
-
-template <int NT>
-__global__ __launch_bounds__(NT) void k_bm_probe(const uint32_t* __restrict__ P, uint64_t n,
-                                                 const uint32_t* __restrict__ bm, uint64_t words,
-                                                 uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
-                                                 uint64_t* __restrict__ out64, uint32_t* __restrict__ out32) {
+// One workgroup per run of `per` buckets [v0, v1) of 2^bits values:
+// dynamic LDS holds the bucket bitmap (2^(bits-5) words) and the bucket
+// bounds of both sets.
+template <int NT, int U>
+__global__ __launch_bounds__(NT) void k_bm_buckets(const uint32_t* __restrict__ S, uint64_t nS,
+                                                   const uint32_t* __restrict__ Q, uint64_t nQ, uint32_t bits,
+                                                   uint64_t nb, uint32_t per, uint64_t* __restrict__ partials,
+                                                   uint32_t* __restrict__ done, uint64_t* __restrict__ out64,
+                                                   uint32_t* __restrict__ out32) {
+    extern __shared__ uint4 dyn[];
     __shared__ uint64_t red[NT / WAVE];
+    uint32_t* bm = reinterpret_cast<uint32_t*>(dyn);
+    const uint32_t words = 1u << (bits - 5), low = (1u << bits) - 1u;
+    uint64_t* bnd = reinterpret_cast<uint64_t*>(bm + words);  // S bounds [0, nbd), Q bounds [nbd, 2 nbd)
+    const uint64_t v0 = uint64_t(blockIdx.x) * per, v1 = v0 + per < nb ? v0 + per : nb;
     uint64_t c = 0;
-    const uint64_t stride = uint64_t(gridDim.x) * NT;
-    for (uint64_t i = uint64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
-        const uint32_t x = P[i];
-        const uint64_t w = x >> 5;
-        if (w < words) c += (bm[w] >> (x & 31)) & 1u;
+    if (v0 < v1) {
+        const uint32_t nbd = uint32_t(v1 - v0) + 1;
+        for (uint32_t t = threadIdx.x; t < 2 * nbd; t += NT) {
+            const bool isQ = t >= nbd;
+            const uint64_t key = (v0 + (isQ ? t - nbd : t)) << bits;  // first value of the bucket
+            const uint32_t* P = isQ ? Q : S;
+            uint64_t lo = 0, hi = isQ ? nQ : nS;  // first position with a value >= key
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (uint64_t(P[mid]) < key) lo = mid + 1; else hi = mid;
+            }
+            bnd[t] = lo;
+        }
+        __syncthreads();
+        // buckets with elements on both sides; the next one's elements (up to
+        // NT * U of each side) are loaded while the current one is processed
+        auto work = [&](uint32_t k) {
+            while (k + 1 < nbd && (bnd[k] == bnd[k + 1] || bnd[nbd + k] == bnd[nbd + k + 1])) ++k;
+            return k;
+        };
+        auto fetch = [&](uint32_t k, uint32_t (&xs)[U], uint32_t (&xq)[U]) {
+            const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t j = s0 + threadIdx.x + uint64_t(u) * NT;
+                xs[u] = S[j < s1 ? j : s1 - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t j = q0 + threadIdx.x + uint64_t(u) * NT;
+                xq[u] = Q[j < q1 ? j : q1 - 1];
+            }
+        };
+        constexpr uint64_t CAP = uint64_t(NT) * U;
+        uint32_t xs[U], xq[U], ys[U], yq[U];
+        uint32_t k = work(0);
+        if (k + 1 < nbd) fetch(k, xs, xq);
+        while (k + 1 < nbd) {
+            const uint32_t kn = work(k + 1);
+            if (kn + 1 < nbd) fetch(kn, ys, yq);
+            const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
+            for (uint32_t i = threadIdx.x; i < words / 4; i += NT) dyn[i] = make_uint4(0u, 0u, 0u, 0u);
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (s0 + threadIdx.x + uint64_t(u) * NT < s1) atomicOr(&bm[(xs[u] & low) >> 5], 1u << (xs[u] & 31));
+            for (uint64_t i = s0 + CAP + threadIdx.x; i < s1; i += NT) {  // beyond NT * U: a dense bucket
+                const uint32_t x = S[i];
+                atomicOr(&bm[(x & low) >> 5], 1u << (x & 31));
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                c += (q0 + threadIdx.x + uint64_t(u) * NT < q1) ? (bm[(xq[u] & low) >> 5] >> (xq[u] & 31)) & 1u : 0u;
+            for (uint64_t i = q0 + CAP + threadIdx.x; i < q1; i += NT) {
+                const uint32_t x = Q[i];
+                c += (bm[(x & low) >> 5] >> (x & 31)) & 1u;
+            }
+            __syncthreads();  // probes done before the next bucket clears the bitmap
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                xs[u] = ys[u];
+                xq[u] = yq[u];
+            }
+            k = kn;
+        }
     }
     grid_fold_u64<NT>(c, partials, done, out64, out32, red);
 }
@@ -222,24 +296,28 @@ int bitmap_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p
     if (nS == 0 || nQ == 0) {
         return zero_outputs(ctx, d_out, d_out32);
     }
-    const uint64_t words = bitmap_words32(ds);
-    SI_TRY(ctx->ws_aux.ensure(words * 4));
-    uint32_t* bm = ctx->ws_aux.as<uint32_t>();
-    SI_HIP(hipMemsetAsync(bm, 0, words * 4, ctx->stream));
-    const uint64_t maxGrid = (p && p->blocks) ? p->blocks : uint64_t(ctx->num_cus) * 8;
-    {
-        KernelScope ks(ctx, "bitmap_sisi_build");
-        const uint64_t g = std::max<uint64_t>(1, std::min<uint64_t>((nS + NT - 1) / NT, maxGrid));
-        hipLaunchKernelGGL(k_bm_build, dim3(uint32_t(g)), dim3(NT), 0, ctx->stream, ds->d_elems + ds->offsets[s], nS, bm);
-        SI_HIP(hipGetLastError());
-    }
-    const uint64_t g = std::max<uint64_t>(1, std::min<uint64_t>((nQ + NT - 1) / NT, maxGrid));
+    // buckets of 2^bits values holding about 4096 elements of the larger set
+    // (what the workgroup's registers take at once)
+    const uint64_t vmax = uint64_t(std::max(ds->max_elem, ds->universe));
+    uint32_t lu = 0, ln = 0;
+    while ((uint64_t(1) << lu) <= vmax) ++lu;  // values fit in lu bits
+    while ((uint64_t(1) << (ln + 1)) <= nQ) ++ln;
+    int bits = int(lu) - int(ln) + 12;
+    if (p && p->reserved[1]) bits = int(p->reserved[1]);  // tuning sweeps
+    bits = bits < 10 ? 10 : (bits > 18 ? 18 : bits);
+    const uint64_t nb = (vmax >> bits) + 1;
+    const uint64_t maxGrid = (p && p->blocks) ? p->blocks : uint64_t(ctx->num_cus) * 4;
+    const uint32_t per = uint32_t((nb + maxGrid - 1) / maxGrid);
+    const uint64_t g = (nb + per - 1) / per;
+    const size_t lds = (size_t(1) << (bits - 5)) * 4 + size_t(2) * (per + 1) * 8;
+    if (lds > 64 * 1024) return fail(SI_ERR_RANGE, "bitmap_dynamic: too many buckets per workgroup");
     void* ws = nullptr;
     SI_TRY(small_workspace(ctx, 256 + g * 8, &ws));
-    KernelScope ks(ctx, "bitmap_sisi_probe");
-    hipLaunchKernelGGL((k_bm_probe<NT>), dim3(uint32_t(g)), dim3(NT), 0, ctx->stream, ds->d_elems + ds->offsets[q], nQ,
-                       bm, words, reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256), static_cast<uint32_t*>(ws),
-                       d_out, d_out32);
+    KernelScope ks(ctx, "bitmap_sisi_buckets");
+    hipLaunchKernelGGL((k_bm_buckets<NT, 16>), dim3(uint32_t(g)), dim3(NT), lds, ctx->stream, ds->d_elems + ds->offsets[s],
+                       nS, ds->d_elems + ds->offsets[q], nQ, uint32_t(bits), nb, per,
+                       reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256), static_cast<uint32_t*>(ws), d_out,
+                       d_out32);
     SI_HIP(hipGetLastError());
     return SI_OK;
 }

@@ -46,6 +46,20 @@ def test_sisi_sizes(ctx, na, nb, U):
     check_pair(ctx, a, b)
 
 
+def test_sisi_dense_and_clustered(ctx):
+    """Dense value ranges (buckets filled to the brim), clusters separated by
+    empty stretches of the universe, and values at both ends of u32."""
+    ev = np.arange(0, 1 << 21, 2, dtype=np.uint32)
+    th = np.arange(0, 1 << 21, 3, dtype=np.uint32)
+    check_pair(ctx, ev, th)
+    top = np.uint64(2**32 - 1)
+    a = np.unique(np.concatenate([np.arange(0, 5000), np.arange(2**31, 2**31 + 70000, 7),
+                                  np.arange(int(top) - 90000, int(top) + 1)])).astype(np.uint32)
+    b = np.unique(np.concatenate([np.arange(0, 5000, 2), np.arange(2**31, 2**31 + 70000, 5),
+                                  np.arange(int(top) - 40000, int(top) + 1, 3)])).astype(np.uint32)
+    check_pair(ctx, a, b)
+
+
 def test_sisi_skewed_2p24x2p12(ctx):
     """A scaled-down copy of the |A|=2^27, |B|=2^16 size ratio."""
     coll = datasets.sisi_pair(1 << 24, 1 << 13, 2**32, 17)
