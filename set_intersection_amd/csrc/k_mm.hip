@@ -217,14 +217,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                     for (int j = 0; j < 4; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                             f4_operand(fa), f4_operand(fb[j]), acc[i][j], 4, 4, 0, 127, 0, 127);
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                        if (i >= 3 && i < 7)
-                            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
-                        else
-                            __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
-                    }
+                    // (no explicit interleaving: the compiler's own schedule
+                    // measured 8% faster than MFMA/VALU group barriers here)
                 }
 #pragma unroll
                 for (int i = 0; i < 8; ++i) wa[i] = na[i];
@@ -263,15 +257,6 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[j], acc[i][j], 0, 0, 0);
-                    // interleave: one MFMA, then a few VALU of the expansions
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                        if (i >= 3 && i < 7)
-                            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
-                        else
-                            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
-                    }
                 }
 #pragma unroll
                 for (int i = 0; i < 8; ++i) wa[i] = na[i];
