@@ -387,20 +387,35 @@ __device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uin
         uint32_t c1 = c0 + WAVE * VT;
         c1 = c1 < we ? c1 : we;
         c1 = c1 < e63 ? c1 : e63;
-        const uint64_t off = (uint64_t(P.y) << 32) | P.x;
+        // partner of each position: partners have elements, so their starts
+        // strictly increase; the partners starting inside a 64-position step
+        // (usually one or two) are walked with scalar reads, and each lane
+        // keeps the last one starting at or before its position
+        uint32_t curS = uint32_t(__builtin_amdgcn_readfirstlane(int(start)));
+        uint32_t curLo = uint32_t(__builtin_amdgcn_readfirstlane(int(P.x)));
+        uint32_t curHi = uint32_t(__builtin_amdgcn_readfirstlane(int(P.y)));
         uint64_t M[VT];
 #pragma unroll
         for (int v = 0; v < VT; ++v) {
-            const uint32_t pos = c0 + uint32_t(v) * WAVE + lane;
-            // largest lane r with start_r <= pos (every lane takes part)
-            int rl = 0, rh = WAVE - 1;
-#pragma unroll
-            for (int step = 0; step < 6; ++step) {
-                const int mid = (rl + rh + 1) >> 1;
-                if (__shfl(start, mid, WAVE) <= pos) rl = mid; else rh = mid - 1;
+            const uint32_t base = c0 + uint32_t(v) * WAVE, pos = base + lane;
+            uint32_t sr = curS, lo = curLo, hi = curHi;
+            uint64_t m = __ballot(valid && start >= base && start < base + WAVE && start > c0);
+            while (m) {
+                const int r = __builtin_ctzll(m);
+                m &= m - 1;
+                const uint32_t s_r = uint32_t(__builtin_amdgcn_readlane(int(start), r));
+                const uint32_t lo_r = uint32_t(__builtin_amdgcn_readlane(int(P.x), r));
+                const uint32_t hi_r = uint32_t(__builtin_amdgcn_readlane(int(P.y), r));
+                if (pos >= s_r) {
+                    sr = s_r;
+                    lo = lo_r;
+                    hi = hi_r;
+                }
+                curS = s_r;
+                curLo = lo_r;
+                curHi = hi_r;
             }
-            const uint32_t sr = __shfl(start, rl, WAVE);
-            const uint64_t orr = __shfl(off, rl, WAVE);
+            const uint64_t orr = (uint64_t(hi) << 32) | lo;
             uint32_t hit = 0;
             if (pos < c1) hit = probeFn(elems[orr + (pos - sr)]);
             M[v] = __ballot(hit);
