@@ -1133,6 +1133,22 @@ int edge_count(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
         const uint32_t grid = uint32_t(std::min<uint64_t>((P->n_zero + 255) / 256, 1024));
         hipLaunchKernelGGL(k_zero_list, dim3(grid), dim3(256), 0, ctx->stream, P->d_zero, P->n_zero, counts);
     }
+    // the small-row kernel runs on the side stream beside the big-row ones:
+    // a bitmap workgroup holds 128 KiB of LDS and 16 waves of a CU, which
+    // leaves room for wave-kernel workgroups next to it
+    const bool forked = P->n_wave && (P->n_bm || P->n_wg);
+    if (P->n_wave) {
+        hipStream_t s = ctx->stream;
+        if (forked) {
+            SI_HIP(hipEventRecord(ctx->fork, ctx->stream));
+            SI_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+            s = ctx->side;
+        }
+        KernelScope ks(ctx, "edges_rows_wave", s);
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * 8, (P->n_wave + 3) / 4));
+        hipLaunchKernelGGL(k_edges_wave, dim3(grid), dim3(WH_NT), 0, s, P->d_wave, P->n_wave, P->d_queue + 3,
+                           ds->d_elems, ds->d_offsets, P->d_part, counts);
+    }
     if (P->n_bm) {
         KernelScope ks(ctx, "edges_rows_bitmap");
         const uint32_t grid = uint32_t(std::min<uint64_t>(ctx->num_cus, P->n_bm));
@@ -1145,11 +1161,9 @@ int edge_count(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
         hipLaunchKernelGGL((k_edges_wg<PR_NT, PR_WORDS>), dim3(grid), dim3(PR_NT), 0, ctx->stream, P->d_wg + P->n_bm,
                            P->n_wg, P->d_queue + 1, ds->d_elems, ds->d_offsets, P->d_part, bitmap_words(ds), counts);
     }
-    if (P->n_wave) {
-        KernelScope ks(ctx, "edges_rows_wave");
-        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * 8, (P->n_wave + 3) / 4));
-        hipLaunchKernelGGL(k_edges_wave, dim3(grid), dim3(WH_NT), 0, ctx->stream, P->d_wave, P->n_wave, P->d_queue + 3,
-                           ds->d_elems, ds->d_offsets, P->d_part, counts);
+    if (forked) {
+        SI_HIP(hipEventRecord(ctx->join, ctx->side));
+        SI_HIP(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
     }
     if (P->n_merge) {
         KernelScope ks(ctx, "edges_merge_pairs");

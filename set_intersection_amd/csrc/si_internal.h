@@ -49,6 +49,10 @@ void free_batch_plan(BatchPlan* p);
 struct si_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for kernels that run beside the
+    // main stream's; joined back to it through the two events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     hipDeviceProp_t prop{};
     int num_cus = 0;
     // DeviceTimer events
@@ -107,8 +111,10 @@ namespace si_impl {
 struct KernelScope {
     si_context* ctx;
     const char* name;
+    hipStream_t stream;
     hipEvent_t start = nullptr, end = nullptr;
     KernelScope(si_context* c, const char* n);
+    KernelScope(si_context* c, const char* n, hipStream_t s);
     ~KernelScope();
 };
 

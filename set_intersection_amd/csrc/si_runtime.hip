@@ -65,16 +65,18 @@ static void drain_pending(si_context* ctx) {
     ctx->pending.clear();
 }
 
-KernelScope::KernelScope(si_context* c, const char* n) : ctx(c), name(n) {
+KernelScope::KernelScope(si_context* c, const char* n) : KernelScope(c, n, c->stream) {}
+
+KernelScope::KernelScope(si_context* c, const char* n, hipStream_t s) : ctx(c), name(n), stream(s) {
     if (!ctx->kernel_timing) return;
     start = take_event(ctx);
     end = take_event(ctx);
-    if (start) (void)hipEventRecord(start, ctx->stream);
+    if (start) (void)hipEventRecord(start, stream);
 }
 
 KernelScope::~KernelScope() {
     if (!ctx->kernel_timing || !start || !end) return;
-    (void)hipEventRecord(end, ctx->stream);
+    (void)hipEventRecord(end, stream);
     ctx->pending.push_back({name, start, end});
     if (ctx->pending.size() > 4096) drain_pending(ctx);
 }
@@ -181,6 +183,9 @@ int si_open(int device, si_context** out) {
     ctx->device = device;
     hipError_t e = hipGetDeviceProperties(&ctx->prop, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete ctx;
         return fail(SI_ERR_HIP, std::string("si_open: ") + hipGetErrorString(e));
@@ -208,6 +213,10 @@ int si_close(si_context* ctx) {
     ctx->ws_lookback.release();
     ctx->ws_aux.release();
     ctx->ws_aux2.release();
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipEventDestroy(ctx->fork);
+    (void)hipEventDestroy(ctx->join);
+    (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SI_OK;
