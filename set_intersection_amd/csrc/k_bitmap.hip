@@ -307,15 +307,19 @@ int bitmap_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p
     bits = bits < 10 ? 10 : (bits > 18 ? 18 : bits);
     const uint64_t nb = (vmax >> bits) + 1;
     const uint64_t maxGrid = (p && p->blocks) ? p->blocks : uint64_t(ctx->num_cus) * 4;
-    const uint32_t per = uint32_t((nb + maxGrid - 1) / maxGrid);
+    uint64_t per = (nb + maxGrid - 1) / maxGrid;
+    // the bucket bounds share the workgroup's 64 KiB of dynamic LDS with the
+    // bitmap: a small --blocks gets more workgroups than it asked for
+    const size_t bmBytes = (size_t(1) << (bits - 5)) * 4;
+    const uint64_t perMax = (64 * 1024 - bmBytes) / 16 - 1;
+    if (per > perMax) per = perMax;
     const uint64_t g = (nb + per - 1) / per;
-    const size_t lds = (size_t(1) << (bits - 5)) * 4 + size_t(2) * (per + 1) * 8;
-    if (lds > 64 * 1024) return fail(SI_ERR_RANGE, "bitmap_dynamic: too many buckets per workgroup");
+    const size_t lds = bmBytes + size_t(2) * (per + 1) * 8;
     void* ws = nullptr;
     SI_TRY(small_workspace(ctx, 256 + g * 8, &ws));
     KernelScope ks(ctx, "bitmap_sisi_buckets");
     hipLaunchKernelGGL((k_bm_buckets<NT, 16>), dim3(uint32_t(g)), dim3(NT), lds, ctx->stream, ds->d_elems + ds->offsets[s],
-                       nS, ds->d_elems + ds->offsets[q], nQ, uint32_t(bits), nb, per,
+                       nS, ds->d_elems + ds->offsets[q], nQ, uint32_t(bits), nb, uint32_t(per),
                        reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256), static_cast<uint32_t*>(ws), d_out,
                        d_out32);
     SI_HIP(hipGetLastError());

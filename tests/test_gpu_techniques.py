@@ -60,6 +60,18 @@ def test_sisi_dense_and_clustered(ctx):
     check_pair(ctx, a, b)
 
 
+def test_sisi_bitmap_few_blocks(ctx):
+    """bitmap_dynamic with --blocks 1 on a dense pair: the bucket bounds do not
+    fit one workgroup's LDS, so the launch takes more workgroups."""
+    a = np.arange(0, 1 << 25, 2, dtype=np.uint32)  # 2^24 elements: 4096 buckets of 2^13 values
+    b = np.arange(0, 1 << 25, 3, dtype=np.uint32)
+    coll = si.Collection.from_sets([a, b])
+    ds = ctx.upload(coll)
+    exp = oracle.count(a, b)
+    assert ds.pair_count(0, 1, si.TECH_BITMAP_DYNAMIC, si.params(blocks=1)) == exp
+    ds.free()
+
+
 def test_sisi_skewed_2p24x2p12(ctx):
     """A scaled-down copy of the |A|=2^27, |B|=2^16 size ratio."""
     coll = datasets.sisi_pair(1 << 24, 1 << 13, 2**32, 17)
