@@ -1,0 +1,80 @@
+"""The drop-in executables end to end on the GPU: each technique executable
+reads a reference-format .bin, runs on the device and writes the result file,
+which must be byte-identical to the golden file of the reference's writers
+(single-run order, io.hpp:91-102; multi-run order for --partition 3 and 5,
+io.hpp:104-136).  The golden files come from tests/golden/make_fixtures.py
+and are pinned by the CPU executables in test_tools.py."""
+import filecmp
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+BIN = os.path.join(ROOT, "set_intersection_amd", "bin")
+FIXTURES = ["closed_form", "random_k12", "random_k23"]
+
+
+def run_exe(exe, name, out, *extra):
+    args = [os.path.join(BIN, exe), "--input", os.path.join(GOLDEN, name + ".bin"), "--output", str(out), *extra]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (args, r.stdout[-2000:], r.stderr[-2000:])
+    return r
+
+
+def golden(name, partition):
+    return os.path.join(GOLDEN, name + ("" if partition is None else f"_p{partition}") + ".txt")
+
+
+@pytest.mark.parametrize("exe,flags", [("ip", []), ("obs", []), ("obs", ["--warp", "--path"]), ("hi", []),
+                                       ("hi", ["--warp", "--path"]), ("bitmap_dynamic", [])])
+@pytest.mark.parametrize("partition", [None, 3])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_technique_executables(exe, flags, partition, name, tmp_path):
+    out = tmp_path / "r.txt"
+    extra = list(flags) + ([] if partition is None else ["--partition", str(partition)])
+    r = run_exe(exe, name, out, *extra)
+    assert "Dataset characteristics" in r.stdout
+    assert filecmp.cmp(str(out), golden(name, partition), shallow=False)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_bitmap_naive_executable(name, tmp_path):
+    out = tmp_path / "r.txt"
+    run_exe("bitmap_naive", name, out)
+    assert filecmp.cmp(str(out), golden(name, None), shallow=False)
+
+
+@pytest.mark.parametrize("variant", ["auto", "fp4", "i8", "f32", "popcount"])
+@pytest.mark.parametrize("partition", [3, 5])
+@pytest.mark.parametrize("name", ["random_k12", "random_k23"])
+def test_mm_executable(variant, partition, name, tmp_path):
+    """The dense-matrix technique on the fixtures with small universes (the
+    reference's mm allocates partition x (universe + 1) floats)."""
+    out = tmp_path / "r.txt"
+    run_exe("mm", name, out, "--partition", str(partition), "--variant", variant)
+    assert filecmp.cmp(str(out), golden(name, partition), shallow=False)
+
+
+@pytest.mark.parametrize("variant", ["fp4", "f32"])
+def test_mm_executable_refuses_fp32_range(variant, tmp_path):
+    """Universe 2^32 - 1: the FP32-accumulated variants refuse (counts could
+    round) and the executable reports the error."""
+    args = [os.path.join(BIN, "mm"), "--input", os.path.join(GOLDEN, "closed_form.bin"), "--output",
+            str(tmp_path / "r.txt"), "--partition", "3", "--variant", variant]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "2^24" in r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("partition", [None, 3])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_misi_batched_executable(partition, name, tmp_path):
+    out = tmp_path / "r.txt"
+    extra = ["--no-graph"] if partition == 3 else []
+    if partition is not None:
+        extra += ["--partition", str(partition)]
+    run_exe("misi_batched", name, out, *extra)
+    assert filecmp.cmp(str(out), golden(name, partition), shallow=False)
