@@ -97,6 +97,30 @@ def cpu_sisi(coll, nbytes, min_seconds, label):
             "sample": f"{label}, std::set_intersection materialising the list, {reps} reps, {el:.1f} s"}
 
 
+def cpu_paths(coll, universe, min_seconds):
+    """The reference's four CPU techniques counting the same pair, one thread
+    each (a single pair gives the reference's OpenMP loop work for one thread
+    only, std_set_intersection.cpp:63-67): std::set_intersection
+    (std_set_intersection.cpp:68-73), the scalar merge of
+    serial_intersect.cpp:93-113, the SSE block intersection that simd.cpp:77
+    calls, and the bitset of universe + 1 bits of boost_bitset.cpp:60-84.
+    Oracle restatements; input GB/s = 4 (|A| + |B|) per count."""
+    import oracle
+    a, b = coll.set(0), coll.set(1)
+    res = {}
+    for m in ("std", "serial", "simd", "bitset"):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            oracle.count(a, b, m, universe=universe)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= min_seconds or reps >= 50:
+                break
+        res[m] = {"input_GBps": round(4 * (a.size + b.size) * reps / el / 1e9, 3), "reps": reps,
+                  "seconds": round(el, 2)}
+    return res
+
+
 def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
     import oracle
     na, nb, universe = 1 << log2a, 1 << log2b, 1 << log2u
@@ -143,6 +167,9 @@ def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
     log(f"[bench] materialise {ms:.4f} ms/step ({value:.0f} GB/s); merge kernel {k_ms:.4f} ms "
         f"({achieved:.0f} GB/s); counts: {techs}")
     cpu = None if args.no_cpu else cpu_sisi(coll, nbytes, args.cpu_seconds, label)
+    if cpu is not None:
+        cpu["paths"] = cpu_paths(coll, universe - 1, args.cpu_seconds / 4)
+        log(f"[bench] CPU paths (1 thread, count only): {cpu['paths']}")
     rec = base_record(args, value, "GB/s", ms,
                       {"workload": f"SISI intersect-path count+materialise, |A|=2^{log2a} |B|=2^{log2b} sorted "
                                    f"unique uint32 uniform in [0,2^{log2u}), seed {args.seed}",
@@ -377,7 +404,23 @@ def main():
         rec = bench_mm(args, si, datasets)
     else:
         rec = bench_edges(args, si, datasets)
+    if rec.get("cpu_baseline"):
+        rec["cpu_baseline"]["host"] = host_cpu()
     print(json.dumps(rec), flush=True)
+
+
+def host_cpu():
+    """CPU model and the logical CPUs this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "cpus_available": len(os.sched_getaffinity(0))}
 
 
 if __name__ == "__main__":

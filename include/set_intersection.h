@@ -140,6 +140,33 @@ int si_buffer_zero(si_buffer* buf);
  * kernel ("buffer_copy"); its HBM counters calibrate traffic measurements. */
 int si_buffer_copy(si_buffer* dst, const si_buffer* src, uint64_t bytes);
 
+/* ---------------------------------------------------- pipelined transfers -- */
+/* The reference copies each run's counts to the host with a blocking
+ * cudaMemcpy between the run's kernels and the next run's (ip.cu:166-169;
+ * the same step in obs.cu, hi.cu, bitmap_dynamic.cu and
+ * matrix_multiplication.cu), so the GPU idles during every transfer.  These
+ * calls let a driver overlap run i's transfer (and the host's writing of run
+ * i's result lines) with run i+1's kernels.
+ *
+ * si_host_alloc: page-locked host memory (released by si_host_free or at
+ * si_close). */
+int si_host_alloc(si_context* ctx, uint64_t bytes, void** out);
+int si_host_free(si_context* ctx, void* host);
+/* Enqueues a device-to-host copy of `bytes` at `offset` of `buf` on the
+ * context's copy stream, ordered after all work enqueued so far on the
+ * context stream, and returns without waiting.  *start_event / *end_event
+ * (either may be null) receive ids of events bracketing the copy, usable with
+ * si_event_elapsed; the host bytes are final once si_event_wait(end) returns.
+ * Work enqueued later on the context stream does not wait for the copy: do not
+ * write to `buf` again (e.g. by launching another run into it) before the copy
+ * has ended.  `host` should come from si_host_alloc; pageable memory is copied
+ * correctly but without overlap. */
+int si_buffer_download_async(si_buffer* buf, uint64_t offset, void* host, uint64_t bytes,
+                             int32_t* start_event, int32_t* end_event);
+/* Blocks the calling thread until the event (si_event_record,
+ * si_buffer_download_async) has completed. */
+int si_event_wait(si_context* ctx, int32_t event_id);
+
 /* ---------------------------------------------------------------- dataset -- */
 typedef struct si_dataset si_dataset;
 

@@ -71,6 +71,18 @@ class Buffer:
         _check(_abi.lib().si_buffer_download(self.h, offset, out.ctypes.data, out.nbytes))
         return out
 
+    def download_async(self, host, offset=0):
+        """Enqueues a copy of host.nbytes bytes at `offset` into the writable
+        contiguous array `host` on the context's copy stream, after the work
+        enqueued so far (si_buffer_download_async).  Returns the ids of the
+        events bracketing the copy; host is final after ctx.wait(end)."""
+        if not (host.flags.c_contiguous and host.flags.writeable):
+            raise ValueError("host must be a writable contiguous array")
+        e0, e1 = c_int32(), c_int32()
+        _check(_abi.lib().si_buffer_download_async(self.h, offset, host.ctypes.data, host.nbytes, byref(e0),
+                                                    byref(e1)))
+        return e0.value, e1.value
+
     def upload(self, arr, offset=0):
         arr = np.ascontiguousarray(arr)
         _check(_abi.lib().si_buffer_upload(self.h, offset, arr.ctypes.data, arr.nbytes))
@@ -127,6 +139,7 @@ class Context:
     def __init__(self, device=0):
         self.h = c_void_p()
         self._children = weakref.WeakSet()
+        self._host = {}  # host_array data address -> si_host_alloc pointer
         _check(_abi.lib().si_open(device, byref(self.h)))
 
     def info(self):
@@ -149,6 +162,26 @@ class Context:
 
     def reset_events(self):
         _check(_abi.lib().si_event_reset(self.h))
+
+    def wait(self, event_id):
+        """Blocks until the event has completed (si_event_wait)."""
+        _check(_abi.lib().si_event_wait(self.h, event_id))
+
+    def host_array(self, count, dtype=np.uint32):
+        """A numpy array over page-locked host memory (si_host_alloc).  It is
+        valid until host_free(array) or close()."""
+        dtype = np.dtype(dtype)
+        nbytes = max(int(count) * dtype.itemsize, 1)
+        p = c_void_p()
+        _check(_abi.lib().si_host_alloc(self.h, nbytes, byref(p)))
+        raw = (ctypes.c_uint8 * nbytes).from_address(p.value)
+        arr = np.frombuffer(raw, dtype=np.uint8)[: int(count) * dtype.itemsize].view(dtype)
+        self._host[arr.ctypes.data] = p.value
+        return arr
+
+    def host_free(self, arr):
+        p = self._host.pop(arr.ctypes.data)
+        _check(_abi.lib().si_host_free(self.h, c_void_p(p)))
 
     def kernel_timing(self, enable=True):
         _check(_abi.lib().si_kernel_timing(self.h, int(enable)))
@@ -176,6 +209,7 @@ class Context:
                     if isinstance(child, kind):
                         child.free()
             _abi.lib().si_close(self.h)
+            self._host.clear()
             self.h = c_void_p()
 
     def __del__(self):

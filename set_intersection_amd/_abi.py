@@ -73,6 +73,10 @@ SIGNATURES = {
     "si_buffer_upload": (c_int, [P, c_uint64, c_void_p, c_uint64]),
     "si_buffer_zero": (c_int, [P]),
     "si_buffer_copy": (c_int, [P, P, c_uint64]),
+    "si_host_alloc": (c_int, [P, c_uint64, POINTER(c_void_p)]),
+    "si_host_free": (c_int, [P, c_void_p]),
+    "si_buffer_download_async": (c_int, [P, c_uint64, c_void_p, c_uint64, POINTER(c_int32), POINTER(c_int32)]),
+    "si_event_wait": (c_int, [P, c_int32]),
     "si_dataset_upload": (c_int, [P, c_uint32, c_uint32, c_uint64, U32P, U32P, c_int32, POINTER(P)]),
     "si_dataset_free": (c_int, [P]),
     "si_dataset_info": (c_int, [P, U32P, U32P, U64P]),

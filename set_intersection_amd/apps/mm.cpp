@@ -1,7 +1,8 @@
 // mm -- bitmap-GEMM set intersection; drop-in for the reference's
 // src/matrix_multiplication.cu executable (target name `mm`, CMakeLists.txt:152).
 // Same flags (--input, --output, --partition), tables, tile loop (:139-184) and
-// result file (writeResult<T, true>, io.hpp:104-136).  The incidence matrices
+// result file (writeResult<T, true>, io.hpp:104-136), written run by run as
+// the runs end (host/pipeline.hpp).  The incidence matrices
 // are built on the device as bit rows and multiplied on the matrix cores:
 // --variant auto (default) takes the fp4 MFMA path (FP32 accumulation, exact
 // while counts stay below 2^24, as the reference's SGEMM) when the universe is
@@ -15,6 +16,7 @@
 
 #include "cli.hpp"
 #include "format.hpp"
+#include "pipeline.hpp"
 #include "set_intersection.h"
 #include "tiles.hpp"
 #include "timers.hpp"
@@ -93,13 +95,11 @@ int main(int argc, char** argv) {
     }
     si::HostTimer hostTimer;
     si::DeviceTimer deviceTimer(ctx);
-    si::Interval* hostAlloc = hostTimer.add("Allocate memory");
-    std::vector<uint32_t> counts(runs.size() * combinations);
-    si::HostTimer::finish(hostAlloc);
 
     si::EventPair* devAlloc = deviceTimer.add("Device memory allocation");
-    si_buffer* dOut = nullptr;
-    if (si_buffer_alloc(ctx, combinations * 4, &dOut) != SI_OK) return die(ctx);
+    si_buffer* dOut[2] = {nullptr, nullptr};
+    for (auto& b : dOut)
+        if (si_buffer_alloc(ctx, combinations * 4, &b) != SI_OK) return die(ctx);
     si::DeviceTimer::finish(devAlloc);
     si::EventPair* xfer = deviceTimer.add("Transfer to device");
     si_dataset* ds = nullptr;
@@ -108,33 +108,39 @@ int main(int argc, char** argv) {
         return die(ctx);
     si::DeviceTimer::finish(xfer);
 
-    uint64_t iter = 0;
-    for (size_t i = 0; i < tiles.size(); ++i) {
-        for (size_t j = i; j < tiles.size(); ++j) {
-            const si::tile& A = tiles[i];
-            const si::tile& B = tiles[j];
-            si::EventPair* mmEv = deviceTimer.add("Matrix multiplication");
-            if (si_mm_counts_async(ds, si_tile{A.id, A.start, A.end}, si_tile{B.id, B.start, B.end}, v, partition,
-                                   dOut) != SI_OK)
-                return die(ctx);
-            si::DeviceTimer::finish(mmEv);
-            si::EventPair* tr = deviceTimer.add("Transfer result");
-            const uint64_t used = uint64_t(A.length - 1) * partition + B.length;
-            if (si_buffer_download(dOut, 0, counts.data() + iter * combinations, used * 4) != SI_OK) return die(ctx);
-            si::DeviceTimer::finish(tr);
-            ++iter;
-        }
+    // runs in findTilePairs order (tiles i <= j, matrix_multiplication.cu:139-150);
+    // counts reach the host and the file through the multi-run pipeline
+    int st = SI_OK;
+    try {
+        st = si::runPipelined<uint32_t, true>(
+            ctx, runs, partition, dOut, deviceTimer, &hostTimer, output,
+            [&](size_t, const si::tile_pair& run, si_buffer* dst) -> int64_t {
+                const si::tile& A = run.first;
+                const si::tile& B = run.second;
+                si::EventPair* mmEv = deviceTimer.add("Matrix multiplication");
+                if (si_mm_counts_async(ds, si_tile{A.id, A.start, A.end}, si_tile{B.id, B.start, B.end}, v,
+                                       partition, dst) != SI_OK)
+                    return -1;
+                si::DeviceTimer::finish(mmEv);
+                return int64_t(uint64_t(A.length - 1) * partition + B.length);
+            });
+    } catch (const si::FormatError& e) {
+        std::printf("%s\n", e.what());
+        si_close(ctx);
+        return 1;
     }
+    if (st != SI_OK) return die(ctx);
     si::EventPair* fr = deviceTimer.add("Free device memory");
     si_dataset_free(ds);
-    si_buffer_free(dOut);
+    si_buffer_free(dOut[0]);
+    si_buffer_free(dOut[1]);
     si::DeviceTimer::finish(fr);
     si_sync(ctx);
     hostTimer.print();
     deviceTimer.print();
     if (!output.empty()) {
+        // written by the pipeline as the runs ended
         std::printf("Writing result to file %s\n", output.c_str());
-        si::writeResult<uint32_t, true>(runs, partition, counts, output);
         std::printf("Finished\n");
     }
     si_close(ctx);

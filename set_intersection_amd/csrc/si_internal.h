@@ -53,6 +53,11 @@ struct si_context {
     // main stream's; joined back to it through the two events
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // copy stream (default priority) of si_buffer_download_async, ordered
+    // after the context stream through copy_fork
+    hipStream_t copy = nullptr;
+    hipEvent_t copy_fork = nullptr;
+    std::set<void*> host_allocs;  // page-locked host memory from si_host_alloc
     hipDeviceProp_t prop{};
     int num_cus = 0;
     // DeviceTimer events

@@ -557,7 +557,6 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
             else if (lane == uint32_t(__ffsll(__ballot(1)) - 1))
                 cnt = hits;
         } else if (steps) {
-            const uint32_t nAw = uint32_t((nA - ia) < W ? (nA - ia) : W);
             const uint32_t nBw = uint32_t((nB - ib) < W ? (nB - ib) : W);
             uint32_t x = x0, y = y0;
             uint32_t av = x < x1 ? rA[(sa + x) & M] : 0u;

@@ -186,7 +186,14 @@ int si_open(int device, si_context** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->copy_fork, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (ctx->copy) (void)hipStreamDestroy(ctx->copy);
+        if (ctx->join) (void)hipEventDestroy(ctx->join);
+        if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return fail(SI_ERR_HIP, std::string("si_open: ") + hipGetErrorString(e));
     }
@@ -201,6 +208,7 @@ int si_close(si_context* ctx) {
     if (!ctx) return SI_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->copy);
     // handles created from the context die with it
     while (!ctx->edge_plans.empty()) si_edge_plan_free(*ctx->edge_plans.begin());
     while (!ctx->datasets.empty()) si_dataset_free(*ctx->datasets.begin());
@@ -213,7 +221,11 @@ int si_close(si_context* ctx) {
     ctx->ws_lookback.release();
     ctx->ws_aux.release();
     ctx->ws_aux2.release();
+    for (void* h : ctx->host_allocs) (void)hipHostFree(h);
+    ctx->host_allocs.clear();
     (void)hipStreamSynchronize(ctx->side);
+    (void)hipEventDestroy(ctx->copy_fork);
+    (void)hipStreamDestroy(ctx->copy);
     (void)hipEventDestroy(ctx->fork);
     (void)hipEventDestroy(ctx->join);
     (void)hipStreamDestroy(ctx->side);
@@ -242,6 +254,7 @@ int si_device_info_get(si_context* ctx, si_device_info* out) {
 int si_sync(si_context* ctx) {
     if (!ctx) return fail(SI_ERR_ARG, "null context");
     SI_HIP(hipStreamSynchronize(ctx->stream));
+    SI_HIP(hipStreamSynchronize(ctx->copy));
     return SI_OK;
 }
 
@@ -265,9 +278,17 @@ int si_event_elapsed(si_context* ctx, int32_t a, int32_t b, float* ms) {
     return SI_OK;
 }
 
+int si_event_wait(si_context* ctx, int32_t id) {
+    if (!ctx) return fail(SI_ERR_ARG, "null context");
+    if (id < 0 || size_t(id) >= ctx->events.size()) return fail(SI_ERR_ARG, "bad event id");
+    SI_HIP(hipEventSynchronize(ctx->events[id]));
+    return SI_OK;
+}
+
 int si_event_reset(si_context* ctx) {
     if (!ctx) return fail(SI_ERR_ARG, "null context");
     SI_HIP(hipStreamSynchronize(ctx->stream));
+    SI_HIP(hipStreamSynchronize(ctx->copy));
     for (auto e : ctx->events) (void)hipEventDestroy(e);
     ctx->events.clear();
     return SI_OK;
@@ -316,6 +337,7 @@ int si_buffer_free(si_buffer* b) {
     if (!b) return SI_OK;
     b->ctx->buffers.erase(b);
     (void)hipStreamSynchronize(b->ctx->stream);
+    (void)hipStreamSynchronize(b->ctx->copy);  // an asynchronous download may still read it
     (void)hipFree(b->ptr);
     delete b;
     return SI_OK;
@@ -333,6 +355,51 @@ int si_buffer_download(si_buffer* b, uint64_t off, void* host, uint64_t bytes) {
     if (!bytes) return SI_OK;
     SI_HIP(hipMemcpyAsync(host, static_cast<char*>(b->ptr) + off, bytes, hipMemcpyDeviceToHost, b->ctx->stream));
     SI_HIP(hipStreamSynchronize(b->ctx->stream));
+    return SI_OK;
+}
+
+int si_buffer_download_async(si_buffer* b, uint64_t off, void* host, uint64_t bytes, int32_t* start_id,
+                             int32_t* end_id) {
+    if (!b || (!host && bytes)) return fail(SI_ERR_ARG, "null argument");
+    if (off + bytes > b->bytes) return fail(SI_ERR_RANGE, "download beyond buffer end");
+    si_context* ctx = b->ctx;
+    // the copy stream waits for everything enqueued so far on the context stream
+    SI_HIP(hipEventRecord(ctx->copy_fork, ctx->stream));
+    SI_HIP(hipStreamWaitEvent(ctx->copy, ctx->copy_fork, 0));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto& e : ev) {
+        SI_HIP(hipEventCreate(&e));
+        ctx->events.push_back(e);  // the context's event table owns it from here on
+    }
+    SI_HIP(hipEventRecord(ev[0], ctx->copy));
+    if (bytes)
+        SI_HIP(hipMemcpyAsync(host, static_cast<char*>(b->ptr) + off, bytes, hipMemcpyDeviceToHost, ctx->copy));
+    SI_HIP(hipEventRecord(ev[1], ctx->copy));
+    if (start_id) *start_id = int32_t(ctx->events.size() - 2);
+    if (end_id) *end_id = int32_t(ctx->events.size() - 1);
+    return SI_OK;
+}
+
+int si_host_alloc(si_context* ctx, uint64_t bytes, void** out) {
+    if (!ctx || !out) return fail(SI_ERR_ARG, "null argument");
+    *out = nullptr;
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes ? bytes : 16, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return fail(SI_ERR_NOMEM, "hipHostMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+    ctx->host_allocs.insert(p);
+    *out = p;
+    return SI_OK;
+}
+
+int si_host_free(si_context* ctx, void* p) {
+    if (!ctx) return fail(SI_ERR_ARG, "null context");
+    if (!p) return SI_OK;
+    auto it = ctx->host_allocs.find(p);
+    if (it == ctx->host_allocs.end()) return fail(SI_ERR_ARG, "pointer not from si_host_alloc on this context");
+    SI_HIP(hipStreamSynchronize(ctx->copy));  // a pending download may still write it
+    ctx->host_allocs.erase(it);
+    SI_HIP(hipHostFree(p));
     return SI_OK;
 }
 

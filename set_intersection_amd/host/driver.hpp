@@ -5,7 +5,8 @@
 // "Dataset characteristics" and "Launch info" tables, runs from findTilePairs,
 // per-run kernel + result transfer timed under the reference's labels, the
 // "Device Timings (in ms)" table, and writeResult in the single-run or
-// multi-run layout.  All device work goes through the C ABI.
+// multi-run layout.  Several runs go through the transfer/write pipeline of
+// pipeline.hpp.  All device work goes through the C ABI.
 #pragma once
 
 #include <cstdint>
@@ -15,6 +16,7 @@
 
 #include "cli.hpp"
 #include "format.hpp"
+#include "pipeline.hpp"
 #include "set_intersection.h"
 #include "tiles.hpp"
 #include "timers.hpp"
@@ -129,13 +131,14 @@ inline int techniqueMain(int argc, char** argv, const TechniqueSpec& spec) {
     p.buckets = buckets;
     p.bucket_size = bucketSize;
 
-    std::vector<uint32_t> counts(runs.size() == 1 ? combinations : runs.size() * combinations);
     DeviceTimer deviceTimer(ctx);
+    const bool pipelined = runs.size() > 1;
 
     EventPair* devMemAlloc = deviceTimer.add("Device memory allocation");
-    si_buffer* dCounts = nullptr;
-    if (si_buffer_alloc(ctx, std::max<uint64_t>(combinations, 1) * sizeof(uint32_t), &dCounts) != SI_OK)
-        return fail("si_buffer_alloc", ctx);
+    si_buffer* dCounts[2] = {nullptr, nullptr};
+    for (int b = 0; b < (pipelined ? 2 : 1); ++b)
+        if (si_buffer_alloc(ctx, std::max<uint64_t>(combinations, 1) * sizeof(uint32_t), &dCounts[b]) != SI_OK)
+            return fail("si_buffer_alloc", ctx);
     DeviceTimer::finish(devMemAlloc);
 
     EventPair* dataTransfer = deviceTimer.add("Transfer to device");
@@ -145,46 +148,61 @@ inline int techniqueMain(int argc, char** argv, const TechniqueSpec& spec) {
         return fail("si_dataset_upload", ctx);
     DeviceTimer::finish(dataTransfer);
 
-    uint64_t iter = 0;
-    for (auto& run : runs) {
+    // one run: counts of all k(k-1)/2 pairs, written in the single-run layout
+    std::vector<uint32_t> counts;
+    auto launchRun = [&](const tile_pair& run, si_buffer* dst) -> int64_t {
         const tile& A = run.first;
         const tile& B = run.second;
         const bool selfJoin = A.id == B.id;
         const uint32_t numOfSets = selfJoin && runs.size() == 1 ? d.cardinality : partition;
         si_tile tA{A.id, A.start, A.end}, tB{B.id, B.start, B.end};
-
         EventPair* compute = deviceTimer.add(spec.kernelLabel);
-        if (si_run_count_async(ds, spec.technique, &p, tA, tB, numOfSets, dCounts) != SI_OK)
-            return fail("si_run_count_async", ctx);
+        if (si_run_count_async(ds, spec.technique, &p, tA, tB, numOfSets, dst) != SI_OK) return -1;
         DeviceTimer::finish(compute);
-
-        EventPair* countTransfer = deviceTimer.add("Transfer result");
         const uint64_t slots = selfJoin ? uint64_t(numOfSets) * (numOfSets - 1) / 2 : uint64_t(numOfSets) * numOfSets;
-        if (slots && si_buffer_download(dCounts, 0, counts.data() + iter * combinations,
-                                        std::min<uint64_t>(slots, combinations) * sizeof(uint32_t)) != SI_OK)
+        return int64_t(std::min<uint64_t>(slots, combinations));
+    };
+    if (pipelined) {
+        try {
+            if (runPipelined<uint32_t>(ctx, runs, partition, dCounts, deviceTimer, nullptr, output,
+                                       [&](size_t, const tile_pair& run, si_buffer* dst) {
+                                           return launchRun(run, dst);
+                                       }) != SI_OK)
+                return fail("multi-run pipeline", ctx);
+        } catch (const FormatError& e) {
+            std::printf("%s\n", e.what());
+            si_close(ctx);
+            return 1;
+        }
+    } else {
+        counts.resize(combinations);
+        const int64_t used = launchRun(runs[0], dCounts[0]);
+        if (used < 0) return fail("si_run_count_async", ctx);
+        EventPair* countTransfer = deviceTimer.add("Transfer result");
+        if (used && si_buffer_download(dCounts[0], 0, counts.data(), uint64_t(used) * sizeof(uint32_t)) != SI_OK)
             return fail("si_buffer_download", ctx);
         DeviceTimer::finish(countTransfer);
-        ++iter;
     }
 
     EventPair* freeDevMem = deviceTimer.add("Free device memory");
     si_dataset_free(ds);
-    si_buffer_free(dCounts);
+    si_buffer_free(dCounts[0]);
+    si_buffer_free(dCounts[1]);
     DeviceTimer::finish(freeDevMem);
     si_sync(ctx);
     deviceTimer.print();
 
     if (!output.empty()) {
+        // the pipeline has written the multi-run file as the runs ended
         std::printf("Writing result to file %s\n", output.c_str());
-        try {
-            if (runs.size() == 1)
+        if (!pipelined) {
+            try {
                 writeResult(d.cardinality, counts, output);
-            else
-                writeResult(runs, partition, counts, output);
-        } catch (const FormatError& e) {
-            std::printf("%s\n", e.what());
-            si_close(ctx);
-            return 1;
+            } catch (const FormatError& e) {
+                std::printf("%s\n", e.what());
+                si_close(ctx);
+                return 1;
+            }
         }
         std::printf("Finished\n");
     }

@@ -188,6 +188,26 @@ inline void writeResult(uint32_t k, const std::vector<T>& counts, const std::str
     w.close();
 }
 
+// The lines of one run of the multi-run layout (the body of the run loop of
+// io.hpp:104-136); `block` is the run's partition^2 block of counts.
+template <typename T, bool mm = false>
+inline void writeRun(LineWriter& w, const tile_pair& run, uint32_t partition, const T* block) {
+    const tile& A = run.first;
+    const tile& B = run.second;
+    bool selfJoin = (A.id == B.id);
+    for (uint64_t i = A.start; i < A.end; ++i) {
+        for (uint64_t j = (selfJoin ? i + 1 : B.start); j < B.end; ++j) {
+            uint64_t pairOffset;
+            uint64_t li = i - uint64_t(A.id) * partition, lj = j - uint64_t(B.id) * partition;
+            if (selfJoin && !mm)
+                pairOffset = triangular_index(partition, li, lj);
+            else
+                pairOffset = quadratic_index(partition, li, lj);
+            w.line(i + 1, j + 1, block[pairOffset]);
+        }
+    }
+}
+
 // writeResult<T, mm>(runs, partition, counts): io.hpp:104-136 -- multi-run
 // layout, one partition^2 block of counts per run in findTilePairs order.
 template <typename T, bool mm = false>
@@ -196,21 +216,7 @@ inline void writeResult(const std::vector<tile_pair>& runs, uint32_t partition, 
     LineWriter w(output);
     uint64_t iter = 0;
     for (auto& run : runs) {
-        const tile& A = run.first;
-        const tile& B = run.second;
-        bool selfJoin = (A.id == B.id);
-        uint64_t partitionOffset = iter * uint64_t(partition) * partition;
-        for (uint64_t i = A.start; i < A.end; ++i) {
-            for (uint64_t j = (selfJoin ? i + 1 : B.start); j < B.end; ++j) {
-                uint64_t pairOffset;
-                uint64_t li = i - uint64_t(A.id) * partition, lj = j - uint64_t(B.id) * partition;
-                if (selfJoin && !mm)
-                    pairOffset = triangular_index(partition, li, lj);
-                else
-                    pairOffset = quadratic_index(partition, li, lj);
-                w.line(i + 1, j + 1, counts[partitionOffset + pairOffset]);
-            }
-        }
+        writeRun<T, mm>(w, run, partition, counts.data() + iter * uint64_t(partition) * partition);
         ++iter;
     }
     w.close();

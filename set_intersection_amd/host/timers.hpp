@@ -85,6 +85,13 @@ struct DeviceTimer {
         return pairs.back().get();
     }
     static void finish(EventPair* p) { si_event_record(p->ctx, &p->end); }
+    // a pair of events the library recorded itself (si_buffer_download_async)
+    EventPair* addRecorded(const std::string& name, int32_t start, int32_t end) {
+        pairs.emplace_back(new EventPair(name, ctx));
+        pairs.back()->start = start;
+        pairs.back()->end = end;
+        return pairs.back().get();
+    }
     double ms(const EventPair& p) const {
         float v = 0.0f;
         if (p.start < 0 || p.end < 0 || si_event_elapsed(ctx, p.start, p.end, &v) != SI_OK) return 0.0;
