@@ -80,13 +80,11 @@ def test_sisi_skewed_2p24x2p12(ctx):
     check_pair(ctx, a, b)
 
 
-@pytest.mark.parametrize("tech", RUN_TECHS)
-@pytest.mark.parametrize("warp", [False, True])
+# --warp selects a kernel only for obs and hi
+@pytest.mark.parametrize("tech,warp", [(t, False) for t in RUN_TECHS] + [(si.TECH_OBS, True), (si.TECH_HI, True)])
 @pytest.mark.parametrize("name,partition", [("closed_form", 10000), ("closed_form", 3), ("random_k12", 5),
                                             ("random_k23", 10000), ("random_k23", 4)])
 def test_golden_runs(ctx, tech, warp, name, partition):
-    if warp and tech not in (si.TECH_OBS, si.TECH_HI):
-        pytest.skip("--warp only selects a kernel for obs and hi")
     coll = datasets.read_bin(os.path.join(GOLDEN, name + ".bin"))
     ds = ctx.upload(coll)
     got = run_layout_counts(ds, coll, tech, partition, si.params(warp=warp))
