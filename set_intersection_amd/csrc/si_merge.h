@@ -457,6 +457,11 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
 
     const uint32_t* rA = sm.win;
     const uint32_t* rB = sm.win + C::SPAN;
+    // split prediction: A's share of the previous tile (16-bit fixed point);
+    // after a tile in which a bounded search missed, the next kBackoff tiles
+    // search the whole range
+    constexpr uint32_t kBackoff = 8;
+    uint32_t shareQ = 1u << 15, fullLeft = 0;
     while (d < dEnd) {
         const uint64_t rem64 = dEnd - d;
         const uint32_t rem = uint32_t(rem64 < T ? rem64 : T);
@@ -477,7 +482,26 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
             const uint32_t hi0 = dd < capA ? dd : capA;
             bool feas = lo <= hi0;
             const uint32_t hi = feas ? hi0 : lo;
-            const uint32_t r = split_search<M, W / 2>(rA, sa, rB, sb, dd, lo, hi);  // hi - lo <= W - 1
+            // bounded search of +-R around the predicted split; exact unless
+            // it ends on a bound of [L, H] that is not a bound of [lo, hi]
+            // and the predicate there says the split lies beyond (then the
+            // whole range is searched).  For uniform data the split stays
+            // within about 150 of the prediction at the end of a tile.
+            constexpr uint32_t R = 256;
+            uint32_t r;
+            bool missed = false;
+            if (fullLeft == 0) {
+                const uint32_t c0 = (dd * shareQ) >> 16;
+                const uint32_t c = c0 < lo ? lo : (c0 > hi ? hi : c0);
+                const uint32_t L = c > lo + (R - 1) ? c - (R - 1) : lo;
+                const uint32_t H = c + R < hi ? c + R : hi;
+                r = split_search<M, R>(rA, sa, rB, sb, dd, L, H);  // H - L <= 2R - 1
+                missed = (r == L && L > lo && (rA[(sa + L - 1) & M] > rB[(sb + dd - L) & M])) ||
+                         (r == H && H < hi && !(rA[(sa + H) & M] > rB[(sb + dd - 1 - H) & M]));
+                if (missed) r = split_search<M, W / 2>(rA, sa, rB, sb, dd, lo, hi);  // hi - lo <= W - 1
+            } else {
+                r = split_search<M, W / 2>(rA, sa, rB, sb, dd, lo, hi);
+            }
             // the true split may lie left of lo (more than capB from B) only
             // when B continues, or right of hi (more than capA from A) only
             // when A continues: test the predicate just outside
@@ -485,12 +509,17 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
             if (r == hi && hi < dd && aMore) feas = feas && (rA[(sa + hi) & M] > rB[(sb + dd - 1 - hi) & M]);
             sm.ends[tid] = r;
             const uint64_t fb = __ballot(feas);
-            if (lane == 0) sm.wave_feas[tid / WAVE] = uint32_t(__popcll(fb));
+            const bool waveMissed = __ballot(missed) != 0;
+            if (lane == 0) sm.wave_feas[tid / WAVE] = uint32_t(__popcll(fb)) | (waveMissed ? 0x10000u : 0u);
         }
         __syncthreads();
-        uint32_t F = 0;
+        uint32_t F = 0, anyMissed = 0;
 #pragma unroll
-        for (int i = 0; i < NT / WAVE; ++i) F += sm.wave_feas[i];
+        for (int i = 0; i < NT / WAVE; ++i) {
+            F += sm.wave_feas[i] & 0xffffu;
+            anyMissed |= sm.wave_feas[i] >> 16;
+        }
+        fullLeft = anyMissed ? kBackoff : (fullLeft ? fullLeft - 1 : 0u);
         const uint32_t Tc = F * VT < rem ? F * VT : rem;
         const uint32_t ca = sm.ends[F - 1];
         const uint32_t cb = Tc - ca;
@@ -629,6 +658,7 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
 
         // 5. commit the refill
         r.commit(sm);
+        if (Tc) shareQ = uint32_t((uint64_t(ca) << 16) / Tc);
         ia += ca;
         ib += cb;
         d += Tc;

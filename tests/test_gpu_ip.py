@@ -99,6 +99,36 @@ def test_sisi_large_oracle(ctx):
     assert np.array_equal(ds.pair_materialize(0, 1), exp)
 
 
+@pytest.mark.parametrize("max_run", [40, 6000])
+def test_sisi_split_prediction_misses(ctx, max_run):
+    """Alternating runs of A-only and B-only values of random lengths, with a
+    value in both sets after every third run: A's share of the merge path
+    jumps from tile to tile, so splits predicted from the previous tile miss
+    (runs up to 6000, beyond the +-256 bounded search) and the full search and
+    its back-off take over.  Count and materialised list against the oracle,
+    also with the set roles swapped."""
+    rng = np.random.default_rng(max_run)
+    a, b, v = [], [], 0
+    while v < 12_000_000:
+        n = int(rng.integers(1, max_run))
+        (a if len(a) <= len(b) else b).append(np.arange(v, v + n, dtype=np.uint64))
+        v += n
+        if rng.integers(3) == 0:
+            a.append(np.array([v], np.uint64))
+            b.append(np.array([v], np.uint64))
+            v += 1
+    a = np.concatenate(a).astype(np.uint32)
+    b = np.concatenate(b).astype(np.uint32)
+    coll = si.Collection.from_sets([a, b])
+    ds = ctx.upload(coll)
+    exp = oracle.materialize(a, b)
+    assert exp.size > 1000
+    for x, y in ((0, 1), (1, 0)):
+        assert ds.pair_count(x, y) == exp.size
+        assert np.array_equal(ds.pair_materialize(x, y), exp)
+    ds.free()
+
+
 def test_async_buffers_and_repeat(ctx):
     coll = datasets.sisi_pair(300_000, 200_000, 1 << 21, 5)
     ds = ctx.upload(coll)
