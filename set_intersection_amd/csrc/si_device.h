@@ -36,6 +36,22 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan_u32(uint32_t v) {
     return v;
 }
 
+// Exclusive prefix sum over the wave of values below 2^BITS, one bit plane at
+// a time (ballot + mbcnt: no cross-lane data movement): lane l receives the
+// sum over lanes below l, `total` the sum over the wave.
+template <int BITS>
+__device__ __forceinline__ uint32_t wave_excl_scan_bits(uint32_t v, uint32_t& total) {
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        const uint64_t m = __ballot((v >> b) & 1u);
+        pre += __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u)) << b;
+        tot += uint32_t(__popcll(m)) << b;
+    }
+    total = tot;
+    return pre;
+}
+
 // Sum over a workgroup of NT threads; `scratch` holds NT/64 uint64 slots.
 // Every thread receives the total.  Contains two barriers.
 template <int NT>

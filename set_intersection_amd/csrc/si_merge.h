@@ -298,14 +298,17 @@ __device__ uint64_t merge_segment(const uint32_t* __restrict__ base, uint64_t oA
 // --------------------------------------------------------------------------
 // Half-window streaming merge (SISI).  A tile of up to T = NT*VT merged
 // elements consumes about T/2 from each side, so each side keeps only a window
-// of W elements (a power of two a little above T/2): the window holds
-// A[ia, ia + W) at slot (absolute index & (W-1)), and the first PAD slots are
-// mirrored past the end so that a thread's VT+1 consecutive reads need no
-// masking.  A thread's split is feasible when it consumes at most capA
-// elements of A and capB of B (W-1 while at least W remain, else all that
-// remain); feasibility is monotone along the diagonal, so the
-// feasible threads form a prefix and the tile is cut after them.  For random
-// data a tile is cut only when one side runs more than (W - T/2) ahead.
+// of W elements (a power of two a little above T/2).  Element e (absolute
+// index in the dataset) lives at slot e & (W-1); the window of A holds the
+// elements from A[ia] up to a multiple of 4, at least W-3 of them while A
+// goes on, so every refill is a run of whole, aligned 16-byte chunks whose
+// slots hold consumed elements only (no per-element masks).  The first PAD
+// slots are mirrored past the end so that a thread's VT+1 consecutive reads
+// need no masking.  A thread's split is feasible when it consumes at most
+// capA elements of A and capB of B (W-4 while more than W-4 remain, else all
+// that remain); feasibility is monotone along the diagonal, so the feasible
+// threads form a prefix and the tile is cut after them.  For random data a
+// tile is cut only when one side runs more than (W - 4 - T/2) ahead.
 template <int NT, int VT, int WN>
 struct StreamCfg {
     static constexpr uint32_t T = uint32_t(NT) * VT;
@@ -333,58 +336,38 @@ struct StreamSmem {
     uint64_t seg[4];
 };
 
-// One refill of at most T elements over both sides, held in registers from
+// One refill of whole 16-byte chunks over both sides, held in registers from
 // issue to commit.  Every lane issues all CPT loads back to back; chunk
 // numbers past the needed ones are clamped to the last needed chunk (in
 // bounds, not committed).
 template <int NT, int VT, int WN>
 struct StreamLoad {
     using C = StreamCfg<NT, VT, WN>;
-    uint4 v[C::CPT];
-    uint32_t nCA, nCB, phA, phB, lenA, lenB, sA, sB;
+    // a native 4 x u32 vector (the refill stays in registers)
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 v[C::CPT];
+    uint32_t nCA, nCB, sA, sB;  // chunk counts; window slot of each side's first chunk
 
-    __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t a0, uint64_t a1, uint64_t b0,
-                                          uint64_t b1) {
-        const uint4* b4 = reinterpret_cast<const uint4*>(base);
-        lenA = a1 > a0 ? uint32_t(a1 - a0) : 0u;
-        lenB = b1 > b0 ? uint32_t(b1 - b0) : 0u;
-        nCA = lenA ? uint32_t(((a1 + 3) >> 2) - (a0 >> 2)) : 0u;
-        nCB = lenB ? uint32_t(((b1 + 3) >> 2) - (b0 >> 2)) : 0u;
-        phA = uint32_t(a0 & 3);
-        phB = uint32_t(b0 & 3);
-        sA = uint32_t(a0 & ~3ull) & C::M;
-        sB = uint32_t(b0 & ~3ull) & C::M;
-        const uint4* pA = b4 + (a0 >> 2);
-        const uint4* pB = b4 + (b0 >> 2);
+    // chunks [ca0, ca1) of the elements (A side) and [cb0, cb1) (B side), as
+    // absolute 16-byte chunk numbers of `base`
+    __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t ca0, uint64_t ca1, uint64_t cb0,
+                                          uint64_t cb1) {
+        const u32x4* b4 = reinterpret_cast<const u32x4*>(base);
+        nCA = uint32_t(ca1 - ca0);
+        nCB = uint32_t(cb1 - cb0);
+        sA = uint32_t(ca0 * 4u) & C::M;
+        sB = uint32_t(cb0 * 4u) & C::M;
+        const u32x4* pA = b4 + ca0;
+        const u32x4* pB = b4 + cb0;
         const uint32_t tot = nCA + nCB;
         if (tot) {
 #pragma unroll
             for (int j = 0; j < C::CPT; ++j) {
                 uint32_t c = threadIdx.x + uint32_t(j) * NT;
                 c = c < tot ? c : tot - 1;
-                const uint4* p = c < nCA ? pA + c : pB + (c - nCA);
+                const u32x4* p = c < nCA ? pA + c : pB + (c - nCA);
                 v[j] = *p;
             }
-        }
-    }
-
-    // chunk c of a range holds elements rel = 4c - ph + q, needed for
-    // 0 <= rel < len; slots below PAD are written to their mirror as well
-    __device__ __forceinline__ static void put(uint32_t* w, const uint4& x, uint32_t c, uint32_t ph, uint32_t len,
-                                               uint32_t s0) {
-        const int rel = int(4 * c) - int(ph);
-        const uint32_t slot = (s0 + 4 * c) & C::M;
-        if (rel >= 0 && rel + 4 <= int(len)) {
-            *reinterpret_cast<uint4*>(&w[slot]) = x;
-            if (slot < C::PAD) *reinterpret_cast<uint4*>(&w[C::W + slot]) = x;
-        } else {
-            const uint32_t e[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (rel + q >= 0 && rel + q < int(len)) {
-                    w[slot + q] = e[q];
-                    if (slot < C::PAD) w[C::W + slot + q] = e[q];
-                }
         }
     }
 
@@ -393,10 +376,13 @@ struct StreamLoad {
 #pragma unroll
         for (int j = 0; j < C::CPT; ++j) {
             const uint32_t c = threadIdx.x + uint32_t(j) * NT;
-            if (c < nCA)
-                put(sm.win, v[j], c, phA, lenA, sA);
-            else if (c < nCA + nCB)
-                put(sm.win + C::SPAN, v[j], c - nCA, phB, lenB, sB);
+            if (c < nCA + nCB) {
+                const bool onB = c >= nCA;
+                uint32_t* w = sm.win + (onB ? C::SPAN : 0u);
+                const uint32_t slot = ((onB ? sB : sA) + 4u * (onB ? c - nCA : c)) & C::M;
+                *reinterpret_cast<u32x4*>(&w[slot]) = v[j];
+                if (slot < C::PAD) *reinterpret_cast<u32x4*>(&w[C::W + slot]) = v[j];
+            }
         }
     }
 };
@@ -437,20 +423,28 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
     constexpr uint32_t T = C::T, W = C::W, M = C::M;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
-    uint64_t myCount = 0;
+    // a lane adds at most VT per tile: 2^25 over any segment of 2^33 elements
+    uint32_t myCount = 0;
     uint64_t written = 0;
 
     uint64_t d = ia + ib;
     const uint64_t dEnd = iaEnd + ibEnd;
     if (d >= dEnd) return 0;
 
+    // window ends in 16-byte chunks: the A window holds the elements
+    // [oA + ia, 4 xA4); the chunk after each set's last element bounds them
+    const uint64_t endA4 = (oA + nA + 3) >> 2, endB4 = (oB + nB + 3) >> 2;
+    auto winEnd4 = [](uint64_t first, uint64_t end4) {
+        const uint64_t e = (first + W) >> 2;
+        return e < end4 ? e : end4;
+    };
+    uint64_t xA4 = winEnd4(oA + ia, endA4), xB4 = winEnd4(oB + ib, endB4);
     {
+        // elements of the first chunk before A[ia] land in slots outside the window
         StreamLoad<NT, VT, WN> r;
-        const uint64_t a1 = ia + W < nA ? ia + W : nA;
-        const uint64_t b1 = ib + W < nB ? ib + W : nB;
-        r.issue(base, oA + ia, oA + a1, 0, 0);
+        r.issue(base, (oA + ia) >> 2, xA4, 0, 0);
         r.commit(sm);
-        r.issue(base, 0, 0, oB + ib, oB + b1);
+        r.issue(base, 0, 0, (oB + ib) >> 2, xB4);
         r.commit(sm);
     }
     __syncthreads();
@@ -466,12 +460,12 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
         const uint64_t rem64 = dEnd - d;
         const uint32_t rem = uint32_t(rem64 < T ? rem64 : T);
         // a side is cut when the window cannot hold all that remains of it;
-        // a tile then consumes at most W-1 of its elements, so every head the
-        // merge reads (index <= W-1) is a true element
-        const bool aMore = (nA - ia) >= W;
-        const bool bMore = (nB - ib) >= W;
-        const uint32_t capA = aMore ? W - 1 : uint32_t(nA - ia);
-        const uint32_t capB = bMore ? W - 1 : uint32_t(nB - ib);
+        // a tile then consumes at most W-4 of its elements, so every head the
+        // merge reads (index <= W-4) is a true element
+        const bool aMore = (nA - ia) > W - 4;
+        const bool bMore = (nB - ib) > W - 4;
+        const uint32_t capA = aMore ? W - 4 : uint32_t(nA - ia);
+        const uint32_t capB = bMore ? W - 4 : uint32_t(nB - ib);
         const uint32_t sa = uint32_t(oA + ia) & M;  // window slot of A[ia]
         const uint32_t sb = uint32_t(oB + ib) & M;
 
@@ -519,9 +513,12 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
             F += sm.wave_feas[i] & 0xffffu;
             anyMissed |= sm.wave_feas[i] >> 16;
         }
+        // the tile's shape is the same in every lane: keep it in scalar registers
+        F = __builtin_amdgcn_readfirstlane(F);
+        anyMissed = __builtin_amdgcn_readfirstlane(anyMissed);
         fullLeft = anyMissed ? kBackoff : (fullLeft ? fullLeft - 1 : 0u);
         const uint32_t Tc = F * VT < rem ? F * VT : rem;
-        const uint32_t ca = sm.ends[F - 1];
+        const uint32_t ca = __builtin_amdgcn_readfirstlane(sm.ends[F - 1]);
         const uint32_t cb = Tc - ca;
         const bool active = tid < F;
         const uint32_t ds = active ? (tid * VT < Tc ? tid * VT : Tc) : Tc;
@@ -533,13 +530,8 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
 
         // 2. fetch the elements that replace the consumed ones (registers)
         StreamLoad<NT, VT, WN> r;
-        {
-            const uint64_t a0 = ia + W < nA ? ia + W : nA;
-            const uint64_t a1 = ia + W + ca < nA ? ia + W + ca : nA;
-            const uint64_t b0 = ib + W < nB ? ib + W : nB;
-            const uint64_t b1 = ib + W + cb < nB ? ib + W + cb : nB;
-            r.issue(base, oA + a0, oA + a1, oB + b0, oB + b1);
-        }
+        const uint64_t nxA4 = winEnd4(oA + ia + ca, endA4), nxB4 = winEnd4(oB + ib + cb, endB4);
+        r.issue(base, xA4, nxA4, xB4, nxB4);
 
         // 3. merge this thread's steps; A is taken on ties and matched against
         //    the current B head (which may lie past y1)
@@ -549,7 +541,7 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
         uint32_t mbits = 0, tbits = 0;
         const uint32_t steps = de - ds;
         if (aMore && bMore && steps == uint32_t(VT)) {
-            // x1 <= W-1 and y1 <= W-1: every head read is a true element and
+            // x1 <= W-4 and y1 <= W-4: every head read is a true element and
             // a thread's reads stay within VT+1 slots of its (masked) start.
             // A[x] at byte pa + 4x, B[ds + k - x] at byte pb + 4k - 4x
             const char* pa = reinterpret_cast<const char*>(rA + (((sa + x0) & M) - x0));
@@ -615,10 +607,14 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
         myCount += cnt;
 
         uint32_t excl = 0, wtot = 0;
+        bool inLds = false;
+        uint32_t* dst = nullptr;
         if (MAT) {
-            const uint32_t incl = wave_inclusive_scan_u32(cnt);
-            excl = incl - cnt;
-            if (lane == WAVE - 1) sm.wave_tot[tid / WAVE] = incl;
+            // cnt <= VT: bit planes of the counts give the wave's prefix
+            constexpr int kBits = 32 - __builtin_clz(uint32_t(VT));
+            uint32_t wsum;
+            excl = wave_excl_scan_bits<kBits>(cnt, wsum);
+            if (lane == 0) sm.wave_tot[tid / WAVE] = wsum;
         }
         __syncthreads();  // the tile's merge reads are done; wave totals are published
 
@@ -634,9 +630,14 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
                 if (i < int(w)) before += t;
                 wtot += t;
             }
+            // uniform in the wave (before) and in the workgroup (wtot)
+            before = __builtin_amdgcn_readfirstlane(before);
+            wtot = __builtin_amdgcn_readfirstlane(wtot);
             // compact in LDS, then one coalesced store; more than HB matches
             // in a tile (dense intersections) go straight to the slot
-            const bool inLds = wtot <= uint32_t(HB);
+            inLds = wtot <= uint32_t(HB);
+            // the tile's part of the slot: a scalar base and 32-bit offsets
+            dst = stage + written;
             uint32_t mb = mbits;
             uint32_t pos = before + excl;
             while (mb) {
@@ -646,18 +647,21 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
                 if (inLds)
                     sm.hits[pos] = v;
                 else
-                    stage[written + pos] = v;
+                    dst[pos] = v;
                 ++pos;
                 mb &= ~(1u << hb);
             }
             __syncthreads();  // matches read from the window before the refill overwrites their slots
-            if (inLds)
-                for (uint32_t i = tid; i < wtot; i += NT) stage[written + i] = sm.hits[i];
             written += wtot;
         }
 
-        // 5. commit the refill
+        // 5. commit the refill; then (MAT) store the tile's compacted matches,
+        //    whose LDS buffer the next tile refills only after two barriers
         r.commit(sm);
+        if (MAT && inLds)
+            for (uint32_t i = tid; i < wtot; i += NT) dst[i] = sm.hits[i];
+        xA4 = nxA4;
+        xB4 = nxB4;
         if (Tc) shareQ = uint32_t((uint64_t(ca) << 16) / Tc);
         ia += ca;
         ib += cb;
