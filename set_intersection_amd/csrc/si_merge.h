@@ -658,8 +658,13 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
         // 5. commit the refill; then (MAT) store the tile's compacted matches,
         //    whose LDS buffer the next tile refills only after two barriers
         r.commit(sm);
-        if (MAT && inLds)
-            for (uint32_t i = tid; i < wtot; i += NT) dst[i] = sm.hits[i];
+        if (MAT && inLds) {
+            // the tile's part of the slot as a buffer: scalar base, 32-bit
+            // offsets (word 3 = 0x00020000: raw 32-bit data on gfx9 parts)
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(dst, 0, int(wtot * 4u), 0x00020000);
+            for (uint32_t i = tid; i < wtot; i += NT)
+                __builtin_amdgcn_raw_buffer_store_b32(sm.hits[i], rs, int(i * 4u), 0, 0);
+        }
         xA4 = nxA4;
         xB4 = nxB4;
         if (Tc) shareQ = uint32_t((uint64_t(ca) << 16) / Tc);
