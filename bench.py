@@ -351,6 +351,16 @@ def bench_edges(args, si, datasets):
                       {"workload": f"R-MAT scale 20, edge factor 16 (a=.57,b=c=.19), undirected, dedup, "
                                    f"seed {args.seed}", "edges": int(m), "total": tot, "triangles": tot // 3})
     rec["plan_seconds"] = round(plan_s, 2)
+    if not args.skip_plain:
+        # the plain baseline on the same plan: one wave per edge, binary search
+        pms, _ = timed(ctx, lambda: plan.count_plain_async(counts, total), 2, 1)
+        pexact = bool(np.array_equal(counts.download(np.uint32), exp) and int(total.download(np.uint64)[0]) == etot)
+        exact = exact and pexact
+        rec["plain"] = {"ms": round(pms, 3), "edges_per_s": round(m / (pms / 1e3), 1),
+                        "kernel": "k_edges_plain: one wave per edge, each element of the shorter list "
+                                  "binary-searched in the longer one", "exact": pexact}
+        rec["speedup_vs_plain"] = round(pms / ms, 2)
+        log(f"[bench] edges batched {ms:.3f} ms, plain {pms:.3f} ms ({pms / ms:.1f}x)")
     rec["exact"] = exact
     rec["cpu_baseline"] = {"value": round(m / cpu_s, 1), "unit": "edges/s", "cores": nth, "kind": "port",
                            "sample": f"all {m} edges, sorted-list intersection per edge, {cpu_s:.1f} s"}

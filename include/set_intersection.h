@@ -295,6 +295,12 @@ int si_edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, ui
 int si_edge_plan_free(si_edge_plan* plan);
 /* counts: m uint32 (device); total: uint64 at byte offset 0 of total_out. */
 int si_edge_count_async(si_edge_plan* plan, si_buffer* counts, si_buffer* total_out);
+/* The plain baseline of si_edge_count_async on the same plan and outputs:
+ * one wavefront per edge, no grouping by vertex, no LDS tables; each element
+ * of the shorter list is binary-searched in the longer list in global memory,
+ * the per-pair search of the reference's obs (obs.cu:330-345) applied to an
+ * explicit pair list. */
+int si_edge_count_plain_async(si_edge_plan* plan, si_buffer* counts, si_buffer* total_out);
 /* One-shot form: plan, run, download, free. */
 int si_edge_count(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m,
                   const si_tuning* tuning, uint32_t* counts_host, uint64_t* total);

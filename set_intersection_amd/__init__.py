@@ -121,6 +121,10 @@ class EdgePlan:
     def count_async(self, counts, total):
         _check(_abi.lib().si_edge_count_async(self.h, counts.h, total.h))
 
+    def count_plain_async(self, counts, total):
+        """The plain per-edge baseline (si_edge_count_plain_async)."""
+        _check(_abi.lib().si_edge_count_plain_async(self.h, counts.h, total.h))
+
     def free(self):
         if self.h and self.ds.h and self.ds.ctx.h:
             _abi.lib().si_edge_plan_free(self.h)

@@ -661,6 +661,50 @@ __global__ __launch_bounds__(NT) void k_sum_counts(const uint32_t* __restrict__ 
     grid_fold_u64<NT>(acc, partials, done, out, nullptr, red);
 }
 
+// Plain edge-wise counts, the baseline the binned dispatcher is measured
+// against: one wave per edge (grid-stride), no grouping by vertex and no LDS
+// tables.  The lanes take the elements of the shorter list and binary-search
+// each in the longer list in global memory (lower bound over the whole list),
+// the per-pair binary search of the reference's obs (obs.cu:330-345 searches
+// each element of one set in the other); the wave sums the matches.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_edges_plain(const uint2* __restrict__ edges, uint64_t m,
+                                                    const uint32_t* __restrict__ elems,
+                                                    const uint64_t* __restrict__ offs, uint32_t* __restrict__ counts,
+                                                    uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
+                                                    uint64_t* __restrict__ total) {
+    __shared__ uint64_t red[NT / WAVE];
+    constexpr uint32_t WPB = NT / WAVE;
+    const uint32_t lane = lane_id();
+    uint64_t acc = 0;
+    for (uint64_t e = uint64_t(blockIdx.x) * WPB + threadIdx.x / WAVE; e < m; e += uint64_t(gridDim.x) * WPB) {
+        const uint2 uv = edges[e];
+        const uint64_t oa = offs[uv.x], na = offs[uv.x + 1] - oa;
+        const uint64_t ob = offs[uv.y], nb = offs[uv.y + 1] - ob;
+        const bool aShort = na <= nb;
+        const uint32_t* S = elems + (aShort ? oa : ob);
+        const uint32_t* L = elems + (aShort ? ob : oa);
+        const uint64_t ns = aShort ? na : nb, nl = aShort ? nb : na;
+        uint64_t hits = 0;
+        if (nl) {
+            const uint64_t top = 1ull << (63 - __builtin_clzll(nl));  // largest power of two <= nl
+            for (uint64_t i = lane; i < ns; i += WAVE) {
+                const uint32_t x = S[i];
+                uint64_t pos = 0;  // elements of L below x
+                for (uint64_t step = top; step; step >>= 1)
+                    if (pos + step <= nl && L[pos + step - 1] < x) pos += step;
+                hits += uint64_t(pos < nl && L[pos] == x);
+            }
+        }
+        const uint64_t c = wave_sum_u64(hits);
+        if (lane == 0) {
+            counts[e] = uint32_t(c);
+            acc += c;
+        }
+    }
+    grid_fold_u64<NT>(acc, partials, done, total, nullptr, red);
+}
+
 __global__ void k_reset_queues(uint32_t* q) {
     if (threadIdx.x < 4) q[threadIdx.x] = 0;
 }
@@ -984,10 +1028,11 @@ struct si_edge_plan {
     uint32_t* d_zero = nullptr;
     uint32_t* d_queue = nullptr;
     uint64_t* d_partials = nullptr;
+    uint2* d_edges = nullptr;  // (u, v) per edge, for the plain baseline
     uint32_t n_bm = 0, n_wg = 0, n_wave = 0, n_merge = 0, n_zero = 0;
     ~si_edge_plan() {
         for (void* p : {(void*)d_wg, (void*)d_wave, (void*)d_part, (void*)d_merge, (void*)d_zero, (void*)d_queue,
-                        (void*)d_partials})
+                        (void*)d_partials, (void*)d_edges})
             if (p) (void)hipFree(p);
     }
 };
@@ -1115,7 +1160,12 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
     SI_HIP(hipMalloc(&plan->d_queue, 256));
     SI_HIP(hipMemsetAsync(plan->d_queue, 0, 256, ctx->stream));
     SI_HIP(hipMalloc(&plan->d_partials, SUM_GRID * 8));
-    SI_HIP(hipStreamSynchronize(ctx->stream));
+    {
+        std::vector<uint2> edges(m);
+        for (uint64_t e = 0; e < m; ++e) edges[e] = make_uint2(u[e], v[e]);
+        SI_TRY(up(edges, plan->d_edges));
+        SI_HIP(hipStreamSynchronize(ctx->stream));  // before `edges` goes out of scope
+    }
     plan->n_wg = uint32_t(wgItems.size());
     plan->n_wave = uint32_t(waveItems.size());
     plan->n_merge = uint32_t(merge.size());
@@ -1177,6 +1227,18 @@ int edge_count(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
         hipLaunchKernelGGL((k_sum_counts<256>), dim3(grid), dim3(256), 0, ctx->stream, counts, P->m, P->d_partials,
                            P->d_queue + 2, total);
     }
+    SI_HIP(hipGetLastError());
+    return SI_OK;
+}
+
+int edge_count_plain(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
+    si_dataset* ds = P->ds;
+    si_context* ctx = ds->ctx;
+    KernelScope ks(ctx, "edges_plain");
+    // one partial per workgroup: at most SUM_GRID workgroups
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(SUM_GRID, (P->m + 3) / 4)));
+    hipLaunchKernelGGL((k_edges_plain<256>), dim3(grid), dim3(256), 0, ctx->stream, P->d_edges, P->m, ds->d_elems,
+                       ds->d_offsets, counts, P->d_partials, P->d_queue + 2, total);
     SI_HIP(hipGetLastError());
     return SI_OK;
 }

@@ -164,3 +164,35 @@ def test_edges_arbitrary_pairs(ctx):
     got, total = ds.edge_count(u, v)
     exp, etot = oracle.edges(coll, u, v)
     assert np.array_equal(got, exp) and total == etot
+
+
+@pytest.mark.parametrize("case", ["rmat", "arbitrary"])
+def test_edges_plain_baseline(ctx, case):
+    """The plain per-edge kernel (one wave per edge, binary search of the
+    shorter list in the longer) on the same plan: counts and total equal the
+    oracle's and the binned dispatcher's, twice in a row (the total's
+    workgroup ticket resets itself), including self loops, repeated edges and
+    empty lists in the arbitrary pairs."""
+    if case == "rmat":
+        coll = datasets.rmat_collection(12, 16, seed=4)
+        u, v = datasets.edge_list(coll)
+    else:
+        sets = [np.unique(np.random.default_rng(i).integers(0, 2**32, n, dtype=np.uint64)).astype(np.uint32)
+                for i, n in enumerate([0, 1, 5, 64, 65, 4000, 70000, 0, 300])]
+        sets[3] = sets[6][::1000].copy()  # a subset of a long list
+        coll = si.Collection.from_sets(sets)
+        rng = np.random.default_rng(3)
+        u = rng.integers(0, coll.k, 5000).astype(np.uint32)
+        v = rng.integers(0, coll.k, 5000).astype(np.uint32)
+    ds = ctx.upload(coll)
+    plan = ds.edge_plan(u, v)
+    exp, etot = oracle.edges(coll, u, v)
+    counts = ctx.buffer(4 * u.size)
+    total = ctx.buffer(8)
+    for _ in range(2):
+        counts.zero()
+        plan.count_plain_async(counts, total)
+        assert np.array_equal(counts.download(np.uint32), exp)
+        assert int(total.download(np.uint64)[0]) == etot
+    plan.count_async(counts, total)
+    assert np.array_equal(counts.download(np.uint32), exp)
