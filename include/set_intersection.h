@@ -295,6 +295,13 @@ int si_edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, ui
 int si_edge_plan_free(si_edge_plan* plan);
 /* counts: m uint32 (device); total: uint64 at byte offset 0 of total_out. */
 int si_edge_count_async(si_edge_plan* plan, si_buffer* counts, si_buffer* total_out);
+/* Times candidate tunings of the edge dispatcher on this edge list (one pass
+ * of coordinate descent over bitmap_min, chunk sizes, workgroups per CU and
+ * the search ratio; each candidate planned, run once to warm up and once
+ * timed) until budget_ms of wall time is spent, and returns the fastest
+ * record for si_edge_plan_create. */
+int si_edge_autotune(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, double budget_ms,
+                     si_tuning* out);
 /* The plain baseline of si_edge_count_async on the same plan and outputs:
  * one wavefront per edge, no grouping by vertex, no LDS tables; each element
  * of the shorter list is binary-searched in the longer list in global memory,

@@ -315,6 +315,13 @@ class Dataset:
                                         _u32p(counts), byref(total)))
         return counts[: u.size], total.value
 
+    def edge_autotune(self, u, v, budget_ms=5000.0):
+        u = np.ascontiguousarray(u, dtype=np.uint32)
+        v = np.ascontiguousarray(v, dtype=np.uint32)
+        t = si_tuning()
+        _check(_abi.lib().si_edge_autotune(self.h, _u32p(u), _u32p(v), u.size, budget_ms, byref(t)))
+        return t
+
     def edge_plan(self, u, v, tuning=None):
         return EdgePlan(self, u, v, tuning)
 

@@ -98,6 +98,7 @@ SIGNATURES = {
     "si_edge_plan_free": (c_int, [P]),
     "si_edge_count_async": (c_int, [P, P, P]),
     "si_edge_count_plain_async": (c_int, [P, P, P]),
+    "si_edge_autotune": (c_int, [P, U32P, U32P, c_uint64, c_double, POINTER(si_tuning)]),
     "si_edge_count": (c_int, [P, U32P, U32P, c_uint64, POINTER(si_tuning), U32P, U64P]),
 }
 

@@ -24,11 +24,12 @@ int main(int argc, char** argv) {
     si_context* ctx = nullptr;
     if (si_open(0, &ctx) != SI_OK) return die(nullptr);
     std::string input, output;
-    uint32_t repeat = 1;
+    uint32_t repeat = 1, tuneMs = 0;
     si::Options options(argv[0], "Help");
     options.add("input", "Input graph (.bin, one adjacency list per vertex)", &input)
         .add("output", "Output result path", &output)
         .add("repeat", "Run the count N times", &repeat)
+        .add("tune", "Autotune the dispatcher for MS milliseconds before planning (0: default tuning)", &tuneMs)
         .addFlag("help", "Print help");
     try {
         options.parse(argc, argv);
@@ -78,9 +79,22 @@ int main(int argc, char** argv) {
         return die(ctx);
     si::DeviceTimer::finish(xfer);
     si::HostTimer ht;
+    si_tuning tuning{};
+    const bool tuned = tuneMs > 0;
+    if (tuned) {
+        si::Interval* tv = ht.add("Autotune");
+        if (si_edge_autotune(ds, eu.data(), ev.data(), eu.size(), double(tuneMs), &tuning) != SI_OK) return die(ctx);
+        si::HostTimer::finish(tv);
+        si::printTable("Tuning", {{"bitmap_min", si::toText(tuning.bitmap_min)},
+                                  {"chunk_elems", si::toText(tuning.chunk_elems)},
+                                  {"chunk_elems_bitmap", si::toText(tuning.chunk_elems_bitmap)},
+                                  {"probe_wg_per_cu", si::toText(tuning.probe_wg_per_cu)},
+                                  {"ratio_search", si::toText(tuning.ratio_search)}});
+    }
     si::Interval* iv = ht.add("Edge plan");
     si_edge_plan* plan = nullptr;
-    if (si_edge_plan_create(ds, eu.data(), ev.data(), eu.size(), nullptr, &plan) != SI_OK) return die(ctx);
+    if (si_edge_plan_create(ds, eu.data(), ev.data(), eu.size(), tuned ? &tuning : nullptr, &plan) != SI_OK)
+        return die(ctx);
     si::HostTimer::finish(iv);
     si_buffer *dc = nullptr, *dt = nullptr;
     if (si_buffer_alloc(ctx, std::max<size_t>(eu.size(), 1) * 4, &dc) != SI_OK || si_buffer_alloc(ctx, 8, &dt) != SI_OK)

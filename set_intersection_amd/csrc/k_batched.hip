@@ -28,6 +28,7 @@
 //    and the launch sequence (queue reset + kernels) can be captured in a
 //    hipGraph and replayed.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -1241,6 +1242,69 @@ int edge_count_plain(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
                        ds->d_offsets, counts, P->d_partials, P->d_queue + 2, total);
     SI_HIP(hipGetLastError());
     return SI_OK;
+}
+
+int edge_autotune(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, double budget_ms,
+                  si_tuning* best) {
+    si_context* ctx = ds->ctx;
+    batched_default_tuning(ds, best);
+    best->use_graph = 0;
+    if (m == 0) return SI_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto wallMs = [&] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    void* counts = nullptr;
+    uint64_t* total = nullptr;
+    SI_HIP(hipMalloc(&counts, m * 4));
+    SI_HIP(hipMalloc(reinterpret_cast<void**>(&total), 8));
+    hipEvent_t e0, e1;
+    SI_HIP(hipEventCreate(&e0));
+    SI_HIP(hipEventCreate(&e1));
+    // plan with t, one warm run, one timed run
+    auto measure = [&](const si_tuning& t, double& ms) -> int {
+        si_edge_plan* plan = nullptr;
+        SI_TRY(edge_plan_create(ds, u, v, m, &t, &plan));
+        int st = edge_count(plan, static_cast<uint32_t*>(counts), total);
+        if (st == SI_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) st = SI_ERR_HIP;
+        if (st == SI_OK) st = edge_count(plan, static_cast<uint32_t*>(counts), total);
+        if (st == SI_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) st = SI_ERR_HIP;
+        float f = 0.f;
+        if (st == SI_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&f, e0, e1) != hipSuccess))
+            st = fail(SI_ERR_HIP, "edge autotune timing failed");
+        edge_plan_free(plan);
+        ms = f;
+        return st;
+    };
+    double bestMs = 0.0;
+    int st = measure(*best, bestMs);
+    // one pass of coordinate descent over the knobs that shape the edge plan
+    const std::vector<std::pair<uint32_t si_tuning::*, std::vector<uint32_t>>> knobs = {
+        {&si_tuning::bitmap_min, {256, 1024, 2048}},
+        {&si_tuning::chunk_elems_bitmap, {1u << 19, 1u << 21}},
+        {&si_tuning::chunk_elems, {32768, 131072}},
+        {&si_tuning::probe_wg_per_cu, {2, 8}},
+        {&si_tuning::ratio_search, {16, 64}},
+    };
+    for (auto& kv : knobs) {
+        for (uint32_t val : kv.second) {
+            if (st != SI_OK || wallMs() > budget_ms) break;
+            si_tuning t = *best;
+            if (t.*(kv.first) == val) continue;
+            t.*(kv.first) = val;
+            double ms = 0.0;
+            st = measure(t, ms);
+            if (st == SI_OK && ms < bestMs) {
+                bestMs = ms;
+                *best = t;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(counts);
+    (void)hipFree(total);
+    return st;
 }
 
 void edge_plan_free(si_edge_plan* p) {

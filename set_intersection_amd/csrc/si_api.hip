@@ -238,6 +238,12 @@ int si_edge_count_async(si_edge_plan* plan, si_buffer* counts, si_buffer* total_
     return edge_count(plan, static_cast<uint32_t*>(counts->ptr), static_cast<uint64_t*>(total_out->ptr));
 }
 
+int si_edge_autotune(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, double budget_ms,
+                     si_tuning* out) {
+    if (!ds || !out || (m && (!u || !v))) return fail(SI_ERR_ARG, "null argument");
+    return edge_autotune(ds, u, v, m, budget_ms, out);
+}
+
 int si_edge_count_plain_async(si_edge_plan* plan, si_buffer* counts, si_buffer* total_out) {
     if (!plan || !counts || !total_out) return fail(SI_ERR_ARG, "null argument");
     if (total_out->bytes < 8) return fail(SI_ERR_CAPACITY, "total buffer smaller than 8 bytes");

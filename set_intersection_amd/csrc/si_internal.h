@@ -163,6 +163,8 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
                      si_edge_plan** out);
 int edge_count(si_edge_plan* plan, uint32_t* d_counts, uint64_t* d_total);
 int edge_count_plain(si_edge_plan* plan, uint32_t* d_counts, uint64_t* d_total);
+int edge_autotune(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, double budget_ms,
+                  si_tuning* best);
 void edge_plan_free(si_edge_plan* plan);
 uint64_t plan_edges(si_edge_plan* plan);
 si_dataset* plan_dataset(si_edge_plan* plan);

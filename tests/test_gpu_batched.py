@@ -196,3 +196,16 @@ def test_edges_plain_baseline(ctx, case):
         assert int(total.download(np.uint64)[0]) == etot
     plan.count_async(counts, total)
     assert np.array_equal(counts.download(np.uint32), exp)
+
+
+def test_edges_autotune(ctx):
+    """si_edge_autotune returns a valid record within its budget; counts with
+    it equal the oracle's."""
+    coll = datasets.rmat_collection(11, 16, seed=8)
+    u, v = datasets.edge_list(coll)
+    ds = ctx.upload(coll)
+    t = ds.edge_autotune(u, v, budget_ms=300.0)
+    assert t.version == 2 and t.chunk_elems > 0 and t.probe_wg_per_cu > 0
+    got, total = ds.edge_count(u, v, t)
+    exp, etot = oracle.edges(coll, u, v)
+    assert np.array_equal(got, exp) and total == etot
