@@ -72,9 +72,9 @@ typedef struct si_params {
     uint32_t path;          /* --path     obs/hi: merge-path first-level split  */
     uint32_t buckets;       /* --buckets      hi (default 512)                  */
     uint32_t bucket_size;   /* --bucket-size  hi (default 1000)                 */
-    uint32_t reserved[4];   /* tuning sweeps, 0 = default: [0] ip SISI tile
-                             * shape; [1] bitmap_dynamic SISI bucket width
-                             * (log2 values per bucket, 10..18)            */
+    uint32_t reserved[4];   /* tuning sweeps, 0 = automatic: [0] ip SISI
+                             * tile shape; [1] bitmap_dynamic SISI bucket
+                             * width (log2 values per bucket, 10..18)      */
 } si_params;
 
 /* A contiguous range of set ids [start, end) with tile number id; mirrors

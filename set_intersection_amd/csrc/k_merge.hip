@@ -225,14 +225,15 @@ constexpr int RUN_VT = 8;
 namespace si_impl {
 
 // Tile shapes of the SISI merge kernel (threads x elements per thread, window
-// W) and workgroups per CU; shape 0 is the default, the others are selectable
-// through si_params.reserved[0] (count) or SI_SISI_SHAPE (materialisation)
-// for tuning sweeps; shape 1 reads one head per merge step.
+// W) and workgroups per CU; 0 and 6 are chosen automatically (auto_shape), the
+// others are selectable through si_params.reserved[0] (count) or
+// SI_SISI_SHAPE (materialisation) for tuning sweeps; shape 1 reads one head
+// per merge step.
 struct SisiShape {
     int nt, vt, per_cu;
 };
 static const SisiShape kShapes[] = {{256, 31, 4}, {256, 31, 4}, {128, 31, 8}, {256, 15, 8}, {128, 15, 16},
-                                    {512, 15, 4}};
+                                    {512, 15, 4}, {256, 15, 4}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 static void sisi_geometry(si_context* ctx, uint64_t n, const si_params* p, const SisiShape& sh, uint32_t& nseg,
@@ -279,20 +280,31 @@ static void launch_shape(int shape, const SisiLaunch& L) {
         case 3: launch_stream<256, 15, 2048, false, MAT>(L); break;
         case 4: launch_stream<128, 15, 1024, false, MAT>(L); break;
         case 5: launch_stream<512, 15, 4096, false, MAT>(L); break;
+        case 6: launch_stream<256, 15, 4096, false, MAT>(L); break;
         default: launch_stream<256, 31, 4096, false, MAT>(L); break;
     }
 }
 
-static int shape_of(const si_params* p) {
+// The default shape for a pair: 256 x 31 (T = 7936) takes about T/2 of each
+// side per tile; when one set is at least 3x the other a tile would take more
+// than W - 4 of the larger and be cut (threads idle), so 256 x 15 (T = 3840 <=
+// W - 4) is used.  Measured at |A| = 2^27, |B| = 2^16: count 0.261 -> 0.198 ms.
+static int auto_shape(uint64_t nA, uint64_t nB) {
+    const uint64_t lo = nA < nB ? nA : nB, hi = nA < nB ? nB : nA;
+    return hi >= 3 * lo ? 6 : 0;
+}
+
+// si_params.reserved[0]: 0 = automatic, else a shape index
+static int shape_of(const si_params* p, uint64_t nA, uint64_t nB) {
     const int s = p ? int(p->reserved[0]) : 0;
-    return (s >= 0 && s < kNumShapes) ? s : 0;
+    return (s > 0 && s < kNumShapes) ? s : auto_shape(nA, nB);
 }
 
 int ip_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32) {
     si_context* ctx = ds->ctx;
     const uint64_t oA = ds->offsets[a], nA = ds->sizes[a];
     const uint64_t oB = ds->offsets[b], nB = ds->sizes[b];
-    const int shape = shape_of(p);
+    const int shape = shape_of(p, nA, nB);
     uint32_t nseg;
     uint64_t seg_len;
     sisi_geometry(ctx, nA + nB, p, kShapes[shape], nseg, seg_len);
@@ -316,7 +328,7 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
         SI_HIP(hipMemsetAsync(d_count, 0, 8, ctx->stream));
         return SI_OK;
     }
-    const int shape = ctx->sisi_shape >= 0 && ctx->sisi_shape < kNumShapes ? ctx->sisi_shape : 0;
+    const int shape = ctx->sisi_shape > 0 && ctx->sisi_shape < kNumShapes ? ctx->sisi_shape : auto_shape(nA, nB);
     uint32_t nseg;
     uint64_t seg_len;
     sisi_geometry(ctx, nA + nB, nullptr, kShapes[shape], nseg, seg_len);

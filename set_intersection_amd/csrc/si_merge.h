@@ -319,7 +319,10 @@ struct StreamCfg {
     static constexpr int CPT = int((T / 4 + 4 + NT - 1) / NT);      // 16-byte chunks per thread per refill
     static_assert((W & (W - 1)) == 0, "window must be a power of two");
     static_assert(uint32_t(VT) + 1 < W, "window shorter than a thread's range");
-    static_assert(W / 4 + 2 <= uint32_t(CPT) * NT, "initial fill exceeds one load");
+    // the initial fill is exactly W/4 chunks (chunk-aligned window ends), a
+    // refill at most T/4 + 2
+    static_assert(W / 4 <= uint32_t(CPT) * NT, "initial fill exceeds one load");
+    static_assert(T / 4 + 2 <= uint32_t(CPT) * NT, "refill exceeds one load");
     static_assert(VT <= 32, "match bits are kept in one 32-bit word");
 };
 

@@ -202,7 +202,7 @@ def _adversarial_pairs():
     yield np.arange(0, 3_000_000, 3, dtype=np.uint32), np.arange(0, 3_000_000, 5, dtype=np.uint32)
 
 
-SISI_SHAPES = 6  # tile shapes of the SISI merge kernel (si_params.reserved[0])
+SISI_SHAPES = 7  # tile shapes of the SISI merge kernel (si_params.reserved[0]; 0 = automatic)
 
 
 @pytest.mark.parametrize("shape", range(SISI_SHAPES))
