@@ -532,9 +532,11 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
         const uint32_t y1 = de - x1;
 
         // 2. fetch the elements that replace the consumed ones (registers)
+        // the segment's last tile needs no refill (nothing reads the window after it)
+        const bool lastTile = d + Tc >= dEnd;
         StreamLoad<NT, VT, WN> r;
         const uint64_t nxA4 = winEnd4(oA + ia + ca, endA4), nxB4 = winEnd4(oB + ib + cb, endB4);
-        r.issue(base, xA4, nxA4, xB4, nxB4);
+        if (!lastTile) r.issue(base, xA4, nxA4, xB4, nxB4);
 
         // 3. merge this thread's steps; A is taken on ties and matched against
         //    the current B head (which may lie past y1)
@@ -660,7 +662,7 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
 
         // 5. commit the refill; then (MAT) store the tile's compacted matches,
         //    whose LDS buffer the next tile refills only after two barriers
-        r.commit(sm);
+        if (!lastTile) r.commit(sm);
         if (MAT && inLds) {
             // the tile's part of the slot as a buffer: scalar base, 32-bit
             // offsets (word 3 = 0x00020000: raw 32-bit data on gfx9 parts)
