@@ -316,13 +316,8 @@ struct StreamCfg {
     static constexpr uint32_t M = W - 1;
     static constexpr uint32_t PAD = (uint32_t(VT) + 2 + 3) & ~3u;  // mirrored head slots
     static constexpr uint32_t SPAN = W + PAD;                       // LDS words per side
-    static constexpr int CPT = int((T / 4 + 4 + NT - 1) / NT);      // 16-byte chunks per thread per refill
     static_assert((W & (W - 1)) == 0, "window must be a power of two");
     static_assert(uint32_t(VT) + 1 < W, "window shorter than a thread's range");
-    // the initial fill is exactly W/4 chunks (chunk-aligned window ends), a
-    // refill at most T/4 + 2
-    static_assert(W / 4 <= uint32_t(CPT) * NT, "initial fill exceeds one load");
-    static_assert(T / 4 + 2 <= uint32_t(CPT) * NT, "refill exceeds one load");
     static_assert(VT <= 32, "match bits are kept in one 32-bit word");
 };
 
@@ -340,51 +335,58 @@ struct StreamSmem {
 };
 
 // One refill of whole 16-byte chunks over both sides, held in registers from
-// issue to commit.  Every lane issues all CPT loads back to back; chunk
-// numbers past the needed ones are clamped to the last needed chunk (in
-// bounds, not committed).
+// issue to commit.  A tile takes at most W-4 elements of a side, so a side's
+// refill is at most W/4 chunks: each thread owns chunks t + j*NT (j < CPS) of
+// each side.  A side's chunks are read through a buffer resource that ends
+// after its last chunk, so every lane issues all its loads with a constant
+// offset (loads past the end return zeros and are not committed).
 template <int NT, int VT, int WN>
 struct StreamLoad {
     using C = StreamCfg<NT, VT, WN>;
+    static constexpr int CPS = int((C::W / 4 + NT - 1) / NT);  // chunks per thread per side
     // a native 4 x u32 vector (the refill stays in registers)
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    u32x4 v[C::CPT];
+    u32x4 va[CPS], vb[CPS];
     uint32_t nCA, nCB, sA, sB;  // chunk counts; window slot of each side's first chunk
 
     // chunks [ca0, ca1) of the elements (A side) and [cb0, cb1) (B side), as
-    // absolute 16-byte chunk numbers of `base`
+    // absolute 16-byte chunk numbers of `base`; at most W/4 per side
     __device__ __forceinline__ void issue(const uint32_t* __restrict__ base, uint64_t ca0, uint64_t ca1, uint64_t cb0,
                                           uint64_t cb1) {
-        const u32x4* b4 = reinterpret_cast<const u32x4*>(base);
         nCA = uint32_t(ca1 - ca0);
         nCB = uint32_t(cb1 - cb0);
         sA = uint32_t(ca0 * 4u) & C::M;
         sB = uint32_t(cb0 * 4u) & C::M;
-        const u32x4* pA = b4 + ca0;
-        const u32x4* pB = b4 + cb0;
-        const uint32_t tot = nCA + nCB;
-        if (tot) {
+        const int vo = int(threadIdx.x * 16u);
+        // word 3 = 0x00020000: raw 32-bit data on gfx9 parts
+        if (nCA) {
+            const __amdgpu_buffer_rsrc_t ra =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(base) + ca0 * 4u, 0, int(nCA * 16u), 0x00020000);
 #pragma unroll
-            for (int j = 0; j < C::CPT; ++j) {
-                uint32_t c = threadIdx.x + uint32_t(j) * NT;
-                c = c < tot ? c : tot - 1;
-                const u32x4* p = c < nCA ? pA + c : pB + (c - nCA);
-                v[j] = *p;
-            }
+            for (int j = 0; j < CPS; ++j) va[j] = __builtin_amdgcn_raw_buffer_load_b128(ra, vo, j * NT * 16, 0);
+        }
+        if (nCB) {
+            const __amdgpu_buffer_rsrc_t rb =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(base) + cb0 * 4u, 0, int(nCB * 16u), 0x00020000);
+#pragma unroll
+            for (int j = 0; j < CPS; ++j) vb[j] = __builtin_amdgcn_raw_buffer_load_b128(rb, vo, j * NT * 16, 0);
         }
     }
 
     template <typename Smem>
     __device__ __forceinline__ void commit(Smem& sm) const {
 #pragma unroll
-        for (int j = 0; j < C::CPT; ++j) {
+        for (int j = 0; j < CPS; ++j) {
             const uint32_t c = threadIdx.x + uint32_t(j) * NT;
-            if (c < nCA + nCB) {
-                const bool onB = c >= nCA;
-                uint32_t* w = sm.win + (onB ? C::SPAN : 0u);
-                const uint32_t slot = ((onB ? sB : sA) + 4u * (onB ? c - nCA : c)) & C::M;
-                *reinterpret_cast<u32x4*>(&w[slot]) = v[j];
-                if (slot < C::PAD) *reinterpret_cast<u32x4*>(&w[C::W + slot]) = v[j];
+            if (c < nCA) {
+                const uint32_t slot = (sA + 4u * c) & C::M;
+                *reinterpret_cast<u32x4*>(&sm.win[slot]) = va[j];
+                if (slot < C::PAD) *reinterpret_cast<u32x4*>(&sm.win[C::W + slot]) = va[j];
+            }
+            if (c < nCB) {
+                const uint32_t slot = (sB + 4u * c) & C::M;
+                *reinterpret_cast<u32x4*>(&sm.win[C::SPAN + slot]) = vb[j];
+                if (slot < C::PAD) *reinterpret_cast<u32x4*>(&sm.win[C::SPAN + C::W + slot]) = vb[j];
             }
         }
     }
