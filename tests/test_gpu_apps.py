@@ -78,3 +78,32 @@ def test_misi_batched_executable(partition, name, tmp_path):
         extra += ["--partition", str(partition)]
     run_exe("misi_batched", name, out, *extra)
     assert filecmp.cmp(str(out), golden(name, partition), shallow=False)
+
+
+@pytest.fixture(scope="module")
+def check_writers(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("cw") / "check_writers")
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "set_intersection_amd", "host"), "-o",
+                        exe, os.path.join(ROOT, "tests", "cpp", "check_writers.cpp")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+@pytest.mark.parametrize("exe", ["ip", "obs"])
+def test_pipelined_many_runs(exe, check_writers, tmp_path):
+    """28 runs (2000 Zipf sets, partition 300) through the executables' transfer
+    and writing pipeline: the file equals the one the host writers produce from
+    counts computed on the CPU (tests/cpp/check_writers.cpp)."""
+    from set_intersection_amd import datasets
+    coll = datasets.zipf_collection(2000, 1.2, 4, 3000, 1 << 16, 21)
+    src = str(tmp_path / "z.bin")
+    datasets.write_bin(coll, src)
+    exp = str(tmp_path / "exp.txt")
+    r = subprocess.run([check_writers, src, exp, "300"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "got.txt"
+    args = [os.path.join(BIN, exe), "--input", src, "--output", str(out), "--partition", "300"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    assert filecmp.cmp(str(out), exp, shallow=False)
