@@ -818,26 +818,46 @@ static int build_items(si_dataset* ds, const si_tuning& t) {
     si_context* ctx = ds->ctx;
     const uint32_t kne = P->kne;
     const bool bmFits = bitmap_words(ds) <= uint32_t(BM_WORDS);
-    std::vector<Item> bm, pr;
     std::vector<uint4> merge;
-    auto emit = [&](std::vector<Item>& v, uint32_t row, uint32_t j0, uint32_t j1, uint32_t mode, uint64_t chunk) {
-        // split [j0, j1) at set boundaries into ~chunk partner elements
-        uint32_t j = j0;
-        while (j < j1) {
+    // A row's partners are the sets after it in size order, cut into items of
+    // about `chunk` partner elements at set boundaries of one grid shared by
+    // all rows, so that the items of different rows over the same grid chunk
+    // read the same partner elements.  The items are ordered chunk-major: the
+    // consecutive items that the resident workgroups pop together share
+    // their partner elements in L2 instead of each streaming its own.
+    auto grid = [&](uint64_t chunk) {
+        std::vector<uint32_t> g{0};
+        while (g.back() < kne) {
+            const uint32_t j = g.back();
             const uint64_t target = P->soff[j] + chunk;
-            uint32_t e = uint32_t(std::upper_bound(P->soff.begin() + j + 1, P->soff.begin() + j1 + 1, target) -
+            uint32_t e = uint32_t(std::upper_bound(P->soff.begin() + j + 1, P->soff.begin() + kne + 1, target) -
                                   P->soff.begin()) - 1;
             if (e <= j) e = j + 1;  // one set larger than the chunk
-            v.push_back(Item{row, j, e, mode});
+            g.push_back(e);
+        }
+        return g;
+    };
+    const std::vector<uint32_t> gBm = grid(t.chunk_elems_bitmap), gPr = grid(t.chunk_elems);
+    struct Tagged {
+        uint32_t m;
+        Item it;
+    };
+    std::vector<Tagged> bmT, prT;
+    auto emit = [&](std::vector<Tagged>& v, const std::vector<uint32_t>& g, uint32_t row, uint32_t j0, uint32_t j1,
+                    uint32_t mode) {
+        uint32_t m = uint32_t(std::upper_bound(g.begin(), g.end(), j0) - g.begin()) - 1;  // g[m] <= j0 < g[m+1]
+        for (uint32_t j = j0; j < j1; ++m) {
+            const uint32_t e = std::min(g[m + 1], j1);
+            v.push_back(Tagged{m, Item{row, j, e, mode}});
             j = e;
         }
     };
     for (uint32_t i = 0; i + 1 < kne; ++i) {
         const uint32_t si = P->ssize[i];
         if (bmFits && (si >= t.bitmap_min || si > HASH_MAX)) {
-            emit(bm, i, i + 1, kne, MODE_BITMAP, t.chunk_elems_bitmap);
+            emit(bmT, gBm, i, i + 1, kne, MODE_BITMAP);
         } else if (si <= HASH_MAX) {
-            emit(pr, i, i + 1, kne, MODE_HASH, t.chunk_elems);
+            emit(prT, gPr, i, i + 1, kne, MODE_HASH);
         } else {
             // BIG row: partners with si >= ratio * |S_j| are searched, larger ones merged
             uint32_t j = i + 1;
@@ -847,9 +867,16 @@ static int build_items(si_dataset* ds, const si_tuning& t) {
                 merge.push_back(make_uint4(P->perm[i], P->perm[j], uint32_t(slot), uint32_t(slot >> 32)));
                 ++j;
             }
-            if (j < kne) emit(pr, i, j, kne, MODE_SEARCH, t.chunk_elems);
+            if (j < kne) emit(prT, gPr, i, j, kne, MODE_SEARCH);
         }
     }
+    auto chunkMajor = [](std::vector<Tagged>& v) {
+        std::stable_sort(v.begin(), v.end(), [](const Tagged& a, const Tagged& b) { return a.m < b.m; });
+        std::vector<Item> out(v.size());
+        for (size_t q = 0; q < v.size(); ++q) out[q] = v[q].it;
+        return out;
+    };
+    std::vector<Item> bm = chunkMajor(bmT), pr = chunkMajor(prT);
     auto upload = [&](auto& vec, auto*& dptr, uint64_t& cap) -> int {
         using T = typename std::decay<decltype(vec)>::type::value_type;
         if (vec.size() > cap || !dptr) {
