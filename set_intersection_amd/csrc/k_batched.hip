@@ -53,7 +53,7 @@ constexpr int BM_NT = 1024, BM_WORDS = 32768;  // 128 KiB LDS bitmap: universes 
 constexpr int PR_NT = 256, PR_WORDS = 8192;    // 32 KiB LDS: hash tables of <= 8192 slots
 constexpr uint32_t HASH_MAX = PR_WORDS / 2;    // largest row that fits a hash table
 constexpr int SEARCH_NS = 4096;                // samples cached for SEARCH rows
-constexpr int VT = 8;                          // elements per lane per chunk
+constexpr int VT = 24;                         // elements per lane per chunk (24 loads in flight: 27.6 -> 24.7 ms for MISI 16384 Zipf)
 
 __device__ __forceinline__ uint32_t hslot(uint32_t x, uint32_t logTS) { return (x * 0x9E3779B1u) >> (32 - logTS); }
 
