@@ -207,7 +207,26 @@ __global__ __launch_bounds__(ROWS_NT) void k_bm_build_rows_lds(const uint32_t* _
         __syncthreads();
         const uint64_t o = offsets[s0 + s], n = offsets[s0 + s + 1] - o;
         const uint64_t chunks = (n + WAVE - 1) / WAVE;
-        for (uint64_t c = w; c < chunks; c += uint64_t(nw) * BUILD_UNROLL) or_bits_chunks(base + o, n, c, nw, lrow);
+        if (n < 8 * w64) {
+            // fewer than 4 elements per 32-bit word on average: lanes rarely
+            // share a word, so one LDS atomic OR per element beats combining
+            // the wave's bits first
+            for (uint64_t c = w; c < chunks; c += uint64_t(nw) * BUILD_UNROLL) {
+                uint32_t x[BUILD_UNROLL];
+                bool ok[BUILD_UNROLL];
+#pragma unroll
+                for (int u = 0; u < BUILD_UNROLL; ++u) {
+                    const uint64_t i = (c + uint64_t(u) * nw) * WAVE + lane_id();
+                    ok[u] = i < n;
+                    x[u] = ok[u] ? base[o + i] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < BUILD_UNROLL; ++u)
+                    if (ok[u]) atomicOr(&lrow[x[u] >> 5], 1u << (x[u] & 31));
+            }
+        } else {
+            for (uint64_t c = w; c < chunks; c += uint64_t(nw) * BUILD_UNROLL) or_bits_chunks(base + o, n, c, nw, lrow);
+        }
         __syncthreads();
         uint4* dst = reinterpret_cast<uint4*>(rows + uint64_t(s) * w64);
         for (uint32_t i = threadIdx.x; i < q4; i += ROWS_NT) dst[i] = lrow4[i];
