@@ -613,64 +613,77 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
         }
         myCount += cnt;
 
-        uint32_t excl = 0, wtot = 0;
-        bool inLds = false;
-        uint32_t* dst = nullptr;
+        // MAT: a wave stages its matches in its own HW-entry part of hits
+        // while the window still holds them (the refill commits only after
+        // the next barrier), at wave-local offsets; after the barrier it
+        // copies them to the slot.  A wave with more than HW matches instead
+        // writes them to the slot straight from the window after the barrier,
+        // and one more barrier then keeps the refill from overwriting them.
+        constexpr uint32_t HW = uint32_t(HB) / (NT / WAVE);
+        uint32_t excl = 0, wsum = 0;
         if (MAT) {
             // cnt <= VT: bit planes of the counts give the wave's prefix
             constexpr int kBits = 32 - __builtin_clz(uint32_t(VT));
-            uint32_t wsum;
             excl = wave_excl_scan_bits<kBits>(cnt, wsum);
+            wsum = __builtin_amdgcn_readfirstlane(wsum);
+            if (wsum <= HW) {
+                // step k's A element is x0 + (A steps before k)
+                uint32_t* hw = sm.hits + (tid / WAVE) * HW;
+                uint32_t mb = mbits, pos = excl;
+                while (mb) {
+                    const uint32_t hb = 31u - uint32_t(__builtin_clz(mb));  // earliest remaining match
+                    const uint32_t j = uint32_t(__popc(tbits >> (hb + 1)));
+                    hw[pos++] = rA[(sa + x0 + j) & M];
+                    mb &= ~(1u << hb);
+                }
+            }
             if (lane == 0) sm.wave_tot[tid / WAVE] = wsum;
         }
         __syncthreads();  // the tile's merge reads are done; wave totals are published
 
-        // 4. with MAT, stage this tile's matches: the matched A elements are
-        //    still in the window (the refill lands after the next barrier);
-        //    step k's A element is x0 + (A steps before k)
+        // 4. with MAT, place the waves' matches in the slot
+        uint32_t before = 0;
+        uint32_t* dst = nullptr;
         if (MAT) {
             const uint32_t w = tid / WAVE;
-            uint32_t before = 0;
+            uint32_t wtot = 0, over = 0;
 #pragma unroll
             for (int i = 0; i < NT / WAVE; ++i) {
                 const uint32_t t = sm.wave_tot[i];
                 if (i < int(w)) before += t;
                 wtot += t;
+                over |= uint32_t(t > HW);
             }
-            // uniform in the wave (before) and in the workgroup (wtot)
+            // uniform in the wave (before) and in the workgroup (wtot, over)
             before = __builtin_amdgcn_readfirstlane(before);
             wtot = __builtin_amdgcn_readfirstlane(wtot);
-            // compact in LDS, then one coalesced store; more than HB matches
-            // in a tile (dense intersections) go straight to the slot
-            inLds = wtot <= uint32_t(HB);
+            over = __builtin_amdgcn_readfirstlane(over);
             // the tile's part of the slot: a scalar base and 32-bit offsets
             dst = stage + written;
-            uint32_t mb = mbits;
-            uint32_t pos = before + excl;
-            while (mb) {
-                const uint32_t hb = 31u - uint32_t(__builtin_clz(mb));  // earliest remaining match
-                const uint32_t j = uint32_t(__popc(tbits >> (hb + 1)));
-                const uint32_t v = rA[(sa + x0 + j) & M];
-                if (inLds)
-                    sm.hits[pos] = v;
-                else
-                    dst[pos] = v;
-                ++pos;
-                mb &= ~(1u << hb);
+            if (wsum > HW) {
+                uint32_t mb = mbits, pos = before + excl;
+                while (mb) {
+                    const uint32_t hb = 31u - uint32_t(__builtin_clz(mb));
+                    const uint32_t j = uint32_t(__popc(tbits >> (hb + 1)));
+                    dst[pos++] = rA[(sa + x0 + j) & M];
+                    mb &= ~(1u << hb);
+                }
             }
-            __syncthreads();  // matches read from the window before the refill overwrites their slots
+            if (over) __syncthreads();  // matches read from the window before the refill overwrites their slots
             written += wtot;
         }
 
-        // 5. commit the refill; then (MAT) store the tile's compacted matches,
-        //    whose LDS buffer the next tile refills only after two barriers
+        // 5. commit the refill; then (MAT) each staging wave stores its
+        //    matches, whose LDS part only this wave rewrites (next tile)
         if (!lastTile) r.commit(sm);
-        if (MAT && inLds) {
-            // the tile's part of the slot as a buffer: scalar base, 32-bit
+        if (MAT && wsum <= HW && wsum) {
+            // the wave's part of the slot as a buffer: scalar base, 32-bit
             // offsets (word 3 = 0x00020000: raw 32-bit data on gfx9 parts)
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(dst, 0, int(wtot * 4u), 0x00020000);
-            for (uint32_t i = tid; i < wtot; i += NT)
-                __builtin_amdgcn_raw_buffer_store_b32(sm.hits[i], rs, int(i * 4u), 0, 0);
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc(dst + before, 0, int(wsum * 4u), 0x00020000);
+            const uint32_t* hw = sm.hits + (tid / WAVE) * HW;
+            for (uint32_t i = lane; i < wsum; i += WAVE)
+                __builtin_amdgcn_raw_buffer_store_b32(hw[i], rs, int(i * 4u), 0, 0);
         }
         xA4 = nxA4;
         xB4 = nxB4;
