@@ -200,6 +200,21 @@ def _adversarial_pairs():
     yield big[:500_000].copy(), big[250_000:].copy()
     yield big[::2].copy(), big[1::2].copy()
     yield np.arange(0, 3_000_000, 3, dtype=np.uint32), np.arange(0, 3_000_000, 5, dtype=np.uint32)
+    # runs of shared values (up to one match per two merge steps) alternate
+    # with runs of interleaved disjoint values at random lengths: in one tile
+    # some waves hold few matches and others more than a wave's share of the
+    # tile's match buffer
+    ra, rb, v = [], [], 0
+    while v < 3_000_000:
+        n = int(r.integers(50, 3000))
+        if r.integers(2):
+            ra.append(np.arange(v, v + n))
+            rb.append(np.arange(v, v + n))
+        else:
+            ra.append(np.arange(v, v + 2 * n, 2))
+            rb.append(np.arange(v + 1, v + 2 * n, 2))
+        v += 2 * n
+    yield np.concatenate(ra).astype(np.uint32), np.concatenate(rb).astype(np.uint32)
 
 
 SISI_SHAPES = 7  # tile shapes of the SISI merge kernel (si_params.reserved[0]; 0 = automatic)
