@@ -8,8 +8,12 @@ the matches) of the HBM-resident pair; the timed region covers exactly K steps b
 events.  value = algorithmic bytes per step / seconds per step, with
 bytes = 4(|A|+|B|) + 4|A∩B| + 4.
 
---workload selects one of the other configurations instead (each prints one
-JSON line of its own):
+The default run also measures the other configurations and attaches one
+compact record for each to the same JSON line (keys sisi_skew, misi, mm,
+edges: time, throughput, the plain baseline and speed-up where there is one,
+exactness against the oracle, and a roofline entry); --headline-only skips
+them.  --workload selects one configuration alone instead (its full record is
+the JSON line):
   sisi_small  |A| = |B| = 2^20 from [0, 2^24)
   sisi_skew   |A| = 2^27, |B| = 2^16 from [0, 2^32): ip / obs / hi / bitmap
   misi        16384 sets, Zipf(1.2) sizes in [16, 65536], universe 2^20:
@@ -31,9 +35,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# default SISI merge kernel (tile shape 0 of k_merge.hip): 256 threads x 31
-# elements per thread, 4096-element windows, materialising instantiation
-SISI_KERNEL = "k_sisi_stream<256,31,4096,materialise>"
+
+
+def sisi_kernel(na, nb):
+    """The SISI merge kernel the library picks for a pair: 256 threads x 31
+    elements per thread with 4096-element windows, or 256 x 15 when one set
+    is at least 3x the other (auto_shape in k_merge.hip)."""
+    lo, hi = min(na, nb), max(na, nb)
+    return f"k_sisi_stream<256,{15 if hi >= 3 * lo else 31},4096,materialise>"
+
 FP4_PEAK_TFLOPS = 10100.0  # dense fp4 MFMA, MI355X
 INT8_PEAK_TOPS = 5000.0
 METRIC = "SISI input GB/s vs HBM peak; MISI pairs/sec batched vs plain; exact counts"
@@ -121,7 +131,7 @@ def cpu_paths(coll, universe, min_seconds):
     return res
 
 
-def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
+def bench_sisi(args, si, datasets, log2a, log2b, log2u, label, with_cpu=True):
     import oracle
     na, nb, universe = 1 << log2a, 1 << log2b, 1 << log2u
     log(f"[bench] generating |A|=2^{log2a} |B|=2^{log2b} uniform in [0,2^{log2u}) ...")
@@ -159,14 +169,15 @@ def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
 
     techs = {}
     for name, t, kname in [("ip", si.TECH_IP, "ip_sisi_count"), ("obs", si.TECH_OBS, "obs_sisi_count"),
-                           ("hi", si.TECH_HI, "hi_sisi_count"), ("bitmap_dynamic", si.TECH_BITMAP_DYNAMIC, None)]:
+                           ("hi", si.TECH_HI, "hi_sisi_count"),
+                           ("bitmap_dynamic", si.TECH_BITMAP_DYNAMIC, "bitmap_sisi_buckets")]:
         cms, _ = timed(ctx, lambda t=t: ds.pair_count_async(0, 1, cnt_only, technique=t), args.steps, args.warmup)
         techs[name] = {"ms": round(cms, 4), "input_GBps": round(4 * (na + nb) / (cms / 1e3) / 1e9, 2)}
         if kname:
             techs[name]["kernel_ms"] = round(kavg(ctx, kname) or 0, 4)
     log(f"[bench] materialise {ms:.4f} ms/step ({value:.0f} GB/s); merge kernel {k_ms:.4f} ms "
         f"({achieved:.0f} GB/s); counts: {techs}")
-    cpu = None if args.no_cpu else cpu_sisi(coll, nbytes, args.cpu_seconds, label)
+    cpu = None if (args.no_cpu or not with_cpu) else cpu_sisi(coll, nbytes, args.cpu_seconds, label)
     if cpu is not None:
         cpu["paths"] = cpu_paths(coll, universe - 1, args.cpu_seconds / 4)
         log(f"[bench] CPU paths (1 thread, count only): {cpu['paths']}")
@@ -177,7 +188,7 @@ def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
     rec["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBPS, 4),
                        "traffic": read_traffic(f"k_sisi_stream_materialise_2^{log2a}x2^{log2b}_pmc.md"),
-                       "kernel": SISI_KERNEL, "kernel_ms": round(k_ms, 4)}
+                       "kernel": sisi_kernel(na, nb), "kernel_ms": round(k_ms, 4)}
     # the executables hand host arrays to the library: the same step with the
     # upload (pinned staging + H2D copy + sortedness check) included
     rec["pcie_inclusive"] = {"upload_s": round(upload_s, 4),
@@ -191,7 +202,17 @@ def bench_sisi(args, si, datasets, log2a, log2b, log2u, label):
 
 
 # ------------------------------------------------------------------- MISI --
-def bench_misi(args, si, datasets):
+def probe_bytes(sizes):
+    """Algorithmic bytes of the batched all-pairs dispatcher: every pair is
+    charged to its larger set, whose table is probed with each element of the
+    smaller one, so one launch streams sum over pairs of 4 min(|A|, |B|)
+    bytes of partner elements (the set at rank r of the size-descending order
+    is the partner of the r sets before it)."""
+    s = np.sort(np.asarray(sizes, dtype=np.float64))[::-1]
+    return 4.0 * float(np.dot(np.arange(s.size, dtype=np.float64), s))
+
+
+def bench_misi(args, si, datasets, cpu_rows=4096):
     import oracle
     coll = datasets.make_workload("misi_zipf1.2_16384", args.seed)
     k = coll.k
@@ -203,9 +224,13 @@ def bench_misi(args, si, datasets):
     t_def = ds.default_tuning()
     ds.batched_count_async(out, t_def)  # plan + items
     ctx.sync()
+    ctx.kernel_timing(True)
     ms_def, _ = timed(ctx, lambda: ds.batched_count_async(out, t_def), args.steps, args.warmup)
+    kernels = {n: round(kavg(ctx, n), 4) for n in ("batched_rows_bitmap", "batched_rows_probe",
+                                                   "batched_merge_pairs", "batched_zero_pairs") if kavg(ctx, n)}
+    ctx.kernel_timing(False)
     res = out.download(np.uint32)
-    log(f"[bench] batched default {ms_def:.3f} ms")
+    log(f"[bench] batched default {ms_def:.3f} ms ({kernels})")
     t0 = time.perf_counter()
     t_tuned = ds.autotune(args.tune_budget_ms)
     tune_s = time.perf_counter() - t0
@@ -251,9 +276,9 @@ def bench_misi(args, si, datasets):
     value = pairs / (best_ms / 1e3)
     # CPU baseline: oracle std::set_intersection on a bounded sample of rows
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and cpu_rows:
         nth = min(16, os.cpu_count() or 1)  # one GPU's share of host cores
-        rows = 4096  # about 59 M pairs: 5-15 s of CPU work
+        rows = cpu_rows  # 4096 rows: about 59 M pairs, 5-15 s of CPU work
         sa = np.repeat(np.arange(0, rows, dtype=np.uint32), k)
         sb = np.tile(np.arange(k, dtype=np.uint32), rows)
         keep = sa < sb
@@ -265,9 +290,17 @@ def bench_misi(args, si, datasets):
                       {"workload": "MISI all pairs, 16384 sets, Zipf(1.2) sizes clamped [16,65536], elements "
                                    f"uniform in [0,2^20), seed {args.seed}", "pairs": pairs})
     rec["batched"] = {"default_ms": round(ms_def, 3), "autotuned_ms": round(ms_tuned, 3),
-                      "tuning": t_tuned.as_dict(), "tune_seconds": round(tune_s, 1)}
+                      "tuning": t_tuned.as_dict(), "tune_seconds": round(tune_s, 1), "kernels_ms": kernels}
     rec["plain"] = plain
     rec["speedup_vs_best_plain"] = round(value / best_plain, 2)
+    # partner elements streamed per launch against HBM peak: the row kernels
+    # read them from L2 / MALL / HBM once per probe
+    pb = probe_bytes(coll.sizes)
+    rec["roofline"] = {"bound": "hbm", "achieved": round(pb / (best_ms / 1e3) / 1e9, 2), "peak": HBM_PEAK_GBPS,
+                       "unit": "GB/s", "frac": round(pb / (best_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
+                       "traffic": None, "kernel": "batched dispatcher (k_rows, k_merge_pairs), whole step",
+                       "algorithmic_bytes": pb,
+                       "basis": "4 B per partner element probed, sum over pairs of 4 min(|A|,|B|)"}
     rec["cpu_baseline"] = cpu
     rec["exact"] = exact
     ctx.close()
@@ -306,17 +339,26 @@ def bench_mm(args, si, datasets):
     Kx = (U + 511) // 512 * 512  # the kernels' K: bit rows padded to whole 512-value chunks
     tiles = (k + 127) // 128
     ops_exec = 2.0 * (tiles * (tiles + 1) // 2) * 128 * 128 * Kx  # self tile: upper block triangle
-    best = "fp4" if res["fp4"]["ms"] <= res["i8"]["ms"] else "i8"
-    peak = FP4_PEAK_TFLOPS if best == "fp4" else INT8_PEAK_TOPS
-    tops = ops_exec / (res[best]["kernel_ms"] / 1e3) / 1e12
-    rec = base_record(args, 2.0 * k * k * Kp / (res[best]["ms"] / 1e3) / 1e12, "TOP/s (dense-equivalent)",
-                      res[best]["ms"], {"workload": f"bitmap GEMM, {k} sets over 2^16, density U[1%,10%], "
-                                                    f"seed {args.seed}", "K": Kp, "variant": best},
-                      dtype="fp4 (e2m1), fp32 accumulate" if best == "fp4" else "i8, int32 accumulate")
+    # the int8 / int32-accumulate path (exact below 2^31) is the reported one;
+    # fp4 with FP32 accumulation (exact below 2^24, the bound of the
+    # reference's cublasSgemm) is faster and reported beside it
+    fastest = "fp4" if res["fp4"]["ms"] <= res["i8"]["ms"] else "i8"
+
+    def mfma_roofline(name, peak):
+        tops = ops_exec / (res[name]["kernel_ms"] / 1e3) / 1e12
+        return {"bound": "mfma", "achieved": round(tops, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(tops / peak, 4), "traffic": None, "kernel": f"k_mm_bits<{name}>",
+                "ops_executed": ops_exec}
+
+    rec = base_record(args, 2.0 * k * k * Kp / (res["i8"]["ms"] / 1e3) / 1e12, "TOP/s (dense-equivalent)",
+                      res["i8"]["ms"], {"workload": f"bitmap GEMM, {k} sets over 2^16, density U[1%,10%], "
+                                                    f"seed {args.seed}", "K": Kp, "variant": "i8",
+                                        "fastest_variant": fastest},
+                      dtype="i8, int32 accumulate")
     rec["variants"] = res
-    rec["roofline"] = {"bound": "mfma", "achieved": round(tops, 2), "peak": peak, "unit": "TFLOP/s",
-                       "frac": round(tops / peak, 4), "traffic": None, "kernel": f"k_mm_bits<{best}>",
-                       "ops_executed": ops_exec}
+    rec["roofline"] = mfma_roofline("i8", INT8_PEAK_TOPS)
+    rec["roofline_fp4"] = mfma_roofline("fp4", FP4_PEAK_TFLOPS)
+    rec["speedup_vs_popcount"] = round(res["popcount"]["ms"] / res["i8"]["ms"], 2)
     rec["exact"] = exact
     rec["cpu_baseline"] = {"value": round(2.0 * 64 * k * Kp / cpu_s / 1e12, 6), "unit": "TOP/s (dense-equivalent)",
                            "cores": nth, "kind": "port",
@@ -378,11 +420,47 @@ def bench_edges(args, si, datasets):
                                   "binary-searched in the longer one", "exact": pexact}
         rec["speedup_vs_plain"] = round(pms / ms, 2)
         log(f"[bench] edges batched {ms:.3f} ms, plain {pms:.3f} ms ({pms / ms:.1f}x)")
+    # each edge streams the shorter adjacency list once through the longer
+    # one's LDS table: 4 min(|N(u)|, |N(v)|) bytes per edge
+    pb = 4.0 * float(np.minimum(coll.sizes[u], coll.sizes[v]).sum(dtype=np.float64))
+    rec["roofline"] = {"bound": "hbm", "achieved": round(pb / (ms / 1e3) / 1e9, 2), "peak": HBM_PEAK_GBPS,
+                       "unit": "GB/s", "frac": round(pb / (ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4), "traffic": None,
+                       "kernel": "edge dispatcher (k_edges_wg, k_edges_wave), whole step", "algorithmic_bytes": pb,
+                       "basis": "4 B per partner element probed, sum over edges of 4 min(|N(u)|,|N(v)|)"}
     rec["exact"] = exact
     rec["cpu_baseline"] = {"value": round(m / cpu_s, 1), "unit": "edges/s", "cores": nth, "kind": "port",
                            "sample": f"all {m} edges, sorted-list intersection per edge, {cpu_s:.1f} s"}
     ctx.close()
     return rec
+
+
+SUB_KEYS = ("value", "unit", "ms_per_step", "dtype", "config", "exact", "roofline", "roofline_fp4", "plain",
+            "speedup_vs_best_plain", "speedup_vs_plain", "speedup_vs_popcount", "batched", "variants", "count_only",
+            "autotuned", "default_ms", "cpu_baseline")
+
+
+def attach_subrecords(rec, args, si, datasets):
+    """The other configurations, measured in the same run with fewer steps
+    (their full records come from --workload); each failure is recorded
+    instead of ending the run."""
+    import argparse as _ap
+    sub = _ap.Namespace(**vars(args))
+    sub.steps, sub.warmup = min(args.steps, 10), min(args.warmup, 2)
+    sub.tune_budget_ms = min(args.tune_budget_ms, 1500.0)
+    jobs = [("sisi_skew", lambda: bench_sisi(sub, si, datasets, 27, 16, 32, "full 2^27 x 2^16 pair",
+                                             with_cpu=False)),
+            ("misi", lambda: bench_misi(sub, si, datasets, cpu_rows=1024)),
+            ("mm", lambda: bench_mm(sub, si, datasets)),
+            ("edges", lambda: bench_edges(sub, si, datasets))]
+    for name, fn in jobs:
+        t0 = time.perf_counter()
+        try:
+            r = fn()
+            rec[name] = {k: r[k] for k in SUB_KEYS if k in r and r[k] is not None}
+            rec[name]["steps"] = sub.steps
+        except Exception as e:  # noqa: BLE001 -- reported in the line, the headline stays valid
+            rec[name] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"[bench] {name}: {time.perf_counter() - t0:.1f} s")
 
 
 def main():
@@ -399,6 +477,8 @@ def main():
     ap.add_argument("--tune-budget-ms", type=float, default=3000.0)
     ap.add_argument("--plain-limit-ms", type=float, default=20000.0)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="sisi: skip the sisi_skew / misi / mm / edges records attached to the headline line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel on MIB MiB (traffic calibration under rocprofv3)")
     args = ap.parse_args()
@@ -421,6 +501,8 @@ def main():
         ctx.close()
     if args.workload == "sisi":
         rec = bench_sisi(args, si, datasets, args.log2n, args.log2n, 32, f"full 2^{args.log2n} x 2^{args.log2n} pair")
+        if not args.headline_only:
+            attach_subrecords(rec, args, si, datasets)
     elif args.workload == "sisi_small":
         rec = bench_sisi(args, si, datasets, 20, 20, 24, "full 2^20 x 2^20 pair")
     elif args.workload == "sisi_skew":

@@ -70,7 +70,7 @@ int main(int argc, char** argv) {
                                                {"Universe", si::toText(d.universe)},
                                                {"Total elements", si::toText(d.totalElements)}});
     si::HostTimer hostTimer;
-    std::vector<uint32_t> counts(si::combination(d.cardinality, 2));
+    std::vector<uint32_t> counts(si::num_pairs(d.cardinality));
     si::Interval* iv = hostTimer.add(kLabel);
     oracle_all_pairs(kMethod, d.cardinality, d.universe, d.offsets.data(), d.elements.data(),
                      threads ? threads : 1, counts.data());

@@ -59,7 +59,7 @@ int main(int argc, char** argv) {
     si::printTable("Bitmap Info", {{"Word size", "64"},
                                    {"Bitmap words", si::toText(words)},
                                    {"Required Memory (MB)", si::toText(words * 8 * d.cardinality / 1000000)}});
-    const uint64_t pairs = si::combination(d.cardinality, 2);
+    const uint64_t pairs = si::num_pairs(d.cardinality);
     std::vector<uint32_t> counts(pairs);
     si::DeviceTimer deviceTimer(ctx);
     si::EventPair* alloc = deviceTimer.add("Device memory allocation");

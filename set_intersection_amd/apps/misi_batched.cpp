@@ -182,7 +182,7 @@ int main(int argc, char** argv) {
                                   {"probe_wg_per_cu", si::toText(t.probe_wg_per_cu)},
                                   {"hipGraph", t.use_graph ? "yes" : "no"}});
 
-    const uint64_t pairs = si::combination(d.cardinality, 2);
+    const uint64_t pairs = si::num_pairs(d.cardinality);
     si::EventPair* alloc = deviceTimer.add("Device memory allocation");
     si_buffer* dCounts = nullptr;
     if (si_buffer_alloc(ctx, std::max<uint64_t>(pairs, 1) * 4, &dCounts) != SI_OK) return die(ctx);

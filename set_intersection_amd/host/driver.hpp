@@ -103,7 +103,7 @@ inline int techniqueMain(int argc, char** argv, const TechniqueSpec& spec) {
 
     std::vector<tile_pair> runs = findTilePairs(d.cardinality, partition);
     const uint64_t combinations =
-        runs.size() == 1 ? combination(d.cardinality, 2) : uint64_t(partition) * partition;
+        runs.size() == 1 ? num_pairs(d.cardinality) : uint64_t(partition) * partition;
     const uint64_t outputMemory = runs.size() * combinations * sizeof(uint32_t);
     const uint64_t deviceMemory = sizeof(uint32_t) * (uint64_t(d.cardinality) * 3 + d.totalElements + combinations);
 

@@ -134,7 +134,16 @@ class LineWriter {
         if (!f_) throw FormatError("cannot create '" + path + "'");
         buf_.resize(1 << 20);
     }
-    ~LineWriter() { close(); }
+    // A destructor must not throw: on an early return or unwind a failed final
+    // write is dropped here (the normal path calls close(), which reports it).
+    ~LineWriter() {
+        try {
+            close();
+        } catch (...) {
+            if (f_) std::fclose(f_);
+            f_ = nullptr;
+        }
+    }
     template <typename T>
     void line(uint64_t i, uint64_t j, T value) {
         if (pos_ + 96 > buf_.size()) flush();

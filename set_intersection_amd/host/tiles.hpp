@@ -2,7 +2,7 @@
 //
 // Follows include/util.hpp of the reference:
 //   tile / splitToTiles / findTilePairs   util.hpp:11-52
-//   combination                           util.hpp:54-66
+//   num_pairs (combination(n, 2))         util.hpp:54-66, every call site uses k = 2
 //   triangular_index / quadratic_index    util.hpp:68-74
 //   formatBytes                           util.hpp:76-96
 // The names and the run order are kept because the multi-run result writer
@@ -52,18 +52,9 @@ inline std::vector<tile_pair> findTilePairs(uint32_t cardinality, uint32_t parti
     return pairs;
 }
 
-// Binomial coefficient C(n, k) in 64 bits.
-inline unsigned long long combination(uint32_t n, uint32_t k) {
-    if (k > n) return 0;
-    if (2ull * k > n) k = n - k;
-    if (k == 0) return 1;
-    unsigned long long r = n;
-    for (unsigned long long i = 2; i <= k; ++i) {
-        r *= (n - i + 1);
-        r /= i;
-    }
-    return r;
-}
+// Number of unordered pairs of n sets, n(n-1)/2 in 64 bits: the only binomial
+// the drivers need (combination(n, 2) at every call site of the reference).
+inline uint64_t num_pairs(uint32_t n) { return n < 2 ? 0 : uint64_t(n) * (n - 1) / 2; }
 
 // Row-major index of pair (i, j), i < j, in the strict upper triangle of n x n.
 inline unsigned long long triangular_index(unsigned long long n, unsigned long long i,

@@ -32,7 +32,7 @@ int main(int argc, char** argv) {
         return mergeCount(d.set(a), d.sizes[a], d.set(b), d.sizes[b]);
     };
     if (partition == 0) {
-        std::vector<uint32_t> c(si::combination(d.cardinality, 2));
+        std::vector<uint32_t> c(si::num_pairs(d.cardinality));
         for (uint32_t a = 0; a < d.cardinality; ++a)
             for (uint32_t b = a + 1; b < d.cardinality; ++b) c[si::triangular_index(d.cardinality, a, b)] = cnt(a, b);
         si::writeResult(d.cardinality, c, out);

@@ -27,6 +27,39 @@ def run_layout_counts(ds, coll, technique, partition, p=None):
     return res
 
 
+def tri_start(k, i):
+    """triangular_index(k, i, i + 1): where row i's pairs (i, j > i) begin."""
+    return k * (k - 1) // 2 - (k - i) * (k - i - 1) // 2
+
+
+def run_layout_triangular(ds, coll, technique, partition, p=None):
+    """All pair counts through the multi-run driver layout (as
+    run_layout_counts), reassembled into the single-run triangular order
+    counts[triangular_index(k, i, j)] with row slices instead of per-pair
+    Python loops, for collections of thousands of sets."""
+    k = coll.k
+    tiles = [(t, t * partition, min((t + 1) * partition, k)) for t in range((k + partition - 1) // partition)]
+    n = k if len(tiles) == 1 else partition
+    out = np.zeros(k * (k - 1) // 2, dtype=np.uint32)
+    for x in range(len(tiles)):
+        for y in range(x, len(tiles)):
+            A, B = si.tile(*tiles[x]), si.tile(*tiles[y])
+            c = ds.run_count(technique, A, B, n, p)
+            for i in range(A.start, A.end):
+                li = i - A.id * n
+                if A.id == B.id:
+                    ln = B.end - i - 1
+                    src = tri_start(n, li)
+                    dst = tri_start(k, i)
+                else:
+                    ln = B.end - B.start
+                    src = li * n
+                    dst = tri_start(k, i) + B.start - i - 1
+                if ln > 0:
+                    out[dst:dst + ln] = c[src:src + ln]
+    return out
+
+
 def check_all_pairs(got, coll):
     counts, _ = oracle.all_pairs(coll)
     k = coll.k
