@@ -180,6 +180,12 @@ int si_dataset_upload(si_context* ctx, uint32_t k, uint32_t universe, uint64_t t
                       const uint32_t* sizes, const uint32_t* elements, int32_t validate,
                       si_dataset** out);
 int si_dataset_free(si_dataset* ds);
+/* Ids of the context events si_dataset_upload recorded on the context stream:
+ * before the host-to-device copies, between the copies and the offsets scan,
+ * and after the scan.  With si_event_elapsed they time the reference's
+ * "Transfer to device" and "Compute set offsets" rows separately
+ * (ip.cu:128-137; obs.cu:162-171).  The ids are invalid after si_event_reset. */
+int si_dataset_upload_events(si_dataset* ds, int32_t* transfer_start, int32_t* scan_start, int32_t* scan_end);
 int si_dataset_info(si_dataset* ds, uint32_t* k, uint32_t* universe, uint64_t* total);
 
 /* ------------------------------------------------------------------ SISI -- */

@@ -325,6 +325,13 @@ class Dataset:
     def edge_plan(self, u, v, tuning=None):
         return EdgePlan(self, u, v, tuning)
 
+    def upload_ms(self):
+        """(transfer, offsets scan) milliseconds of the upload, from the
+        events si_dataset_upload recorded (si_dataset_upload_events)."""
+        ev = [c_int32(), c_int32(), c_int32()]
+        _check(_abi.lib().si_dataset_upload_events(self.h, *[byref(e) for e in ev]))
+        return self.ctx.elapsed_ms(ev[0].value, ev[1].value), self.ctx.elapsed_ms(ev[1].value, ev[2].value)
+
     def free(self):
         if self.h:
             if self.ctx.h:

@@ -80,6 +80,7 @@ SIGNATURES = {
     "si_dataset_upload": (c_int, [P, c_uint32, c_uint32, c_uint64, U32P, U32P, c_int32, POINTER(P)]),
     "si_dataset_free": (c_int, [P]),
     "si_dataset_info": (c_int, [P, U32P, U32P, U64P]),
+    "si_dataset_upload_events": (c_int, [P, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "si_pair_count": (c_int, [P, c_uint32, c_uint32, c_int32, POINTER(si_params), U64P]),
     "si_pair_count_async": (c_int, [P, c_uint32, c_uint32, c_int32, POINTER(si_params), P, c_uint64]),
     "si_pair_materialize": (c_int, [P, c_uint32, c_uint32, U32P, c_uint64, U64P]),

@@ -69,14 +69,13 @@ int main(int argc, char** argv) {
         return 1;
     }
     si::DeviceTimer::finish(alloc);
-    si::EventPair* xfer = deviceTimer.add("Transfer to device");
     si_dataset* ds = nullptr;
     if (si_dataset_upload(ctx, d.cardinality, d.universe, d.totalElements, d.sizes.data(), d.elements.data(), 1, &ds) !=
         SI_OK) {
         std::fprintf(stderr, "GPUassert: %s\n", si_last_error());
         return 1;
     }
-    si::DeviceTimer::finish(xfer);
+    deviceTimer.addUpload(ds);
     si_params p{};
     p.blocks = options.count("blocks") ? blocks : 0;
     p.threads = options.count("threads") ? blockSize : 0;

@@ -72,12 +72,11 @@ int main(int argc, char** argv) {
                                              {"Adjacency entries", si::toText(d.totalElements)},
                                              {"Edges (u < v)", si::toText(eu.size())}});
     si::DeviceTimer deviceTimer(ctx);
-    si::EventPair* xfer = deviceTimer.add("Transfer to device");
     si_dataset* ds = nullptr;
     if (si_dataset_upload(ctx, d.cardinality, d.universe, d.totalElements, d.sizes.data(), d.elements.data(), 1, &ds) !=
         SI_OK)
         return die(ctx);
-    si::DeviceTimer::finish(xfer);
+    deviceTimer.addUpload(ds);
     si::HostTimer ht;
     si_tuning tuning{};
     const bool tuned = tuneMs > 0;

@@ -141,12 +141,11 @@ int main(int argc, char** argv) {
                                                {"Universe", si::toText(d.universe)},
                                                {"Total elements", si::toText(d.totalElements)}});
     si::DeviceTimer deviceTimer(ctx);
-    si::EventPair* xfer = deviceTimer.add("Transfer to device");
     si_dataset* ds = nullptr;
     if (si_dataset_upload(ctx, d.cardinality, d.universe, d.totalElements, d.sizes.data(), d.elements.data(), 1, &ds) !=
         SI_OK)
         return die(ctx);
-    si::DeviceTimer::finish(xfer);
+    deviceTimer.addUpload(ds);
 
     const std::string key = shapeKey(d);
     si_tuning t;

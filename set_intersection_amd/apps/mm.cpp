@@ -101,12 +101,11 @@ int main(int argc, char** argv) {
     for (auto& b : dOut)
         if (si_buffer_alloc(ctx, combinations * 4, &b) != SI_OK) return die(ctx);
     si::DeviceTimer::finish(devAlloc);
-    si::EventPair* xfer = deviceTimer.add("Transfer to device");
     si_dataset* ds = nullptr;
     if (si_dataset_upload(ctx, d.cardinality, d.universe, d.totalElements, d.sizes.data(), d.elements.data(), 1, &ds) !=
         SI_OK)
         return die(ctx);
-    si::DeviceTimer::finish(xfer);
+    deviceTimer.addUpload(ds);
 
     // runs in findTilePairs order (tiles i <= j, matrix_multiplication.cu:139-150);
     // counts reach the host and the file through the multi-run pipeline

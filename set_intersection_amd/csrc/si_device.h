@@ -141,6 +141,32 @@ __device__ __forceinline__ uint64_t corank_halfwave(const uint32_t* __restrict__
     return lo;
 }
 
+// corank_halfwave restricted to [lo, hi]: the caller knows the co-rank lies
+// there (e.g. the end of a segment whose start co-rank is known: the end of
+// a segment of L merged elements starting at (ia, ib) lies in
+// [max(ia, d - nB), min(nA, ia + L)]), so the search takes ceil(log32(hi-lo))
+// rounds.  Same calling rules as corank_halfwave.
+__device__ __forceinline__ uint64_t corank_halfwave_range(const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
+                                                          uint64_t d, uint64_t lo, uint64_t hi) {
+    const uint32_t lane = lane_id() & 31u;
+    const uint32_t half = lane_id() >> 5;
+    while (lo < hi) {
+        const uint64_t step = (hi - lo + 31) >> 5;
+        const uint64_t p = lo + uint64_t(lane) * step;
+        const bool pred = p < hi && A[p] > B[d - 1 - p];
+        const uint32_t m = uint32_t(__ballot(pred) >> (32 * half));
+        if (m) {
+            const uint32_t f = __builtin_ctz(m);
+            const uint64_t pf = lo + uint64_t(f) * step;
+            lo = f ? lo + uint64_t(f - 1) * step + 1 : lo;
+            hi = pf;
+        } else {
+            lo = lo + ((hi - 1 - lo) / step) * step + 1;
+        }
+    }
+    return lo;
+}
+
 // Single-thread binary co-rank search (used for short ranges).
 __device__ __forceinline__ uint64_t corank_serial(const uint32_t* __restrict__ A, uint64_t nA,
                                                   const uint32_t* __restrict__ B, uint64_t nB, uint64_t d) {

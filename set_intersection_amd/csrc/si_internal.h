@@ -108,6 +108,7 @@ struct si_dataset {
     uint32_t max_size = 0;
     uint32_t max_elem = 0;          // largest element of the collection
     si_impl::BatchPlan* plan = nullptr;  // batched dispatcher state (built on first use)
+    int32_t upload_events[3] = {-1, -1, -1};  // context event ids: transfers, offsets scan, end
 };
 
 namespace si_impl {

@@ -463,16 +463,20 @@ int si_dataset_upload(si_context* ctx, uint32_t k, uint32_t universe, uint64_t t
     if (hipMalloc(&ds->d_sizes, (uint64_t(k) + 1) * 4) != hipSuccess) return cleanup(SI_ERR_NOMEM, "hipMalloc(sizes) failed");
     if (hipMalloc(&ds->d_offsets, (uint64_t(k) + 1) * 8) != hipSuccess)
         return cleanup(SI_ERR_NOMEM, "hipMalloc(offsets) failed");
+    // events bracketing the transfers and the offsets scan (si_dataset_upload_events)
+    if (si_event_record(ctx, &ds->upload_events[0]) != SI_OK) return cleanup(SI_ERR_HIP, si_last_error());
     hipError_t e = hipMemsetAsync(ds->d_elems + total, 0, 64 * 4, ctx->stream);
     if (e == hipSuccess && total)
         e = hipMemcpyAsync(ds->d_elems, elements, total * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && k) e = hipMemcpyAsync(ds->d_sizes, sizes, uint64_t(k) * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("dataset upload: ") + hipGetErrorString(e));
+    if (si_event_record(ctx, &ds->upload_events[1]) != SI_OK) return cleanup(SI_ERR_HIP, si_last_error());
     {
         KernelScope ks(ctx, "offsets_scan");
         hipLaunchKernelGGL(k_offsets_scan, dim3(1), dim3(1024), 0, ctx->stream, ds->d_sizes, k, ds->d_offsets);
     }
     e = hipGetLastError();
+    if (e == hipSuccess && si_event_record(ctx, &ds->upload_events[2]) != SI_OK) return cleanup(SI_ERR_HIP, si_last_error());
     if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("offsets scan: ") + hipGetErrorString(e));
     // the largest element (sets are sorted: the max of the last elements)
     for (uint32_t i = 0; i < k; ++i)
@@ -513,6 +517,14 @@ int si_dataset_free(si_dataset* ds) {
     if (ds->d_sizes) (void)hipFree(ds->d_sizes);
     if (ds->d_offsets) (void)hipFree(ds->d_offsets);
     delete ds;
+    return SI_OK;
+}
+
+int si_dataset_upload_events(si_dataset* ds, int32_t* transfer_start, int32_t* scan_start, int32_t* scan_end) {
+    if (!ds) return fail(SI_ERR_ARG, "null dataset");
+    if (transfer_start) *transfer_start = ds->upload_events[0];
+    if (scan_start) *scan_start = ds->upload_events[1];
+    if (scan_end) *scan_end = ds->upload_events[2];
     return SI_OK;
 }
 

@@ -141,12 +141,11 @@ inline int techniqueMain(int argc, char** argv, const TechniqueSpec& spec) {
             return fail("si_buffer_alloc", ctx);
     DeviceTimer::finish(devMemAlloc);
 
-    EventPair* dataTransfer = deviceTimer.add("Transfer to device");
     si_dataset* ds = nullptr;
     if (si_dataset_upload(ctx, d.cardinality, d.universe, d.totalElements, d.sizes.data(), d.elements.data(), 1,
                           &ds) != SI_OK)
         return fail("si_dataset_upload", ctx);
-    DeviceTimer::finish(dataTransfer);
+    deviceTimer.addUpload(ds);
 
     // one run: counts of all k(k-1)/2 pairs, written in the single-run layout
     std::vector<uint32_t> counts;

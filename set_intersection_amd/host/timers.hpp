@@ -92,6 +92,14 @@ struct DeviceTimer {
         pairs.back()->end = end;
         return pairs.back().get();
     }
+    // "Transfer to device" and "Compute set offsets" from the events that
+    // si_dataset_upload recorded around its copies and its offsets scan
+    void addUpload(si_dataset* ds) {
+        int32_t ev[3] = {-1, -1, -1};
+        si_dataset_upload_events(ds, &ev[0], &ev[1], &ev[2]);
+        addRecorded("Transfer to device", ev[0], ev[1]);
+        addRecorded("Compute set offsets", ev[1], ev[2]);
+    }
     double ms(const EventPair& p) const {
         float v = 0.0f;
         if (p.start < 0 || p.end < 0 || si_event_elapsed(ctx, p.start, p.end, &v) != SI_OK) return 0.0;

@@ -38,7 +38,19 @@ def test_technique_executables(exe, flags, partition, name, tmp_path):
     extra = list(flags) + ([] if partition is None else ["--partition", str(partition)])
     r = run_exe(exe, name, out, *extra)
     assert "Dataset characteristics" in r.stdout
+    # the reference's device-timer rows for the upload (ip.cu:128-137)
+    assert "Transfer to device" in r.stdout and "Compute set offsets" in r.stdout
     assert filecmp.cmp(str(out), golden(name, partition), shallow=False)
+
+
+def test_upload_events(ctx):
+    """si_dataset_upload_events times the copies and the offsets scan apart."""
+    import set_intersection_amd as si
+    from set_intersection_amd import datasets
+    ds = ctx.upload(datasets.zipf_collection(3000, 1.2, 4, 3000, 1 << 16, 2))
+    xfer, scan = ds.upload_ms()
+    assert xfer > 0.0 and scan > 0.0
+    ds.free()
 
 
 @pytest.mark.parametrize("name", FIXTURES)
