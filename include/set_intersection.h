@@ -69,7 +69,11 @@ typedef struct si_params {
     uint32_t blocks;        /* --blocks   (reference default: 16 x CUs)         */
     uint32_t threads;       /* --threads  (reference default: maxThreads/2)     */
     uint32_t warp;          /* --warp     obs/hi: wave-level instead of block   */
-    uint32_t path;          /* --path     obs/hi: merge-path first-level split  */
+    uint32_t path;          /* --path     obs/hi: cut the merge path of each pair
+                             * into segments and apply the technique per
+                             * segment (intersectPathOBS obs.cu:438-522,
+                             * intersectPathHI hi.cu:420-479); SISI pairs and
+                             * runs; ignored by ip and bitmap_dynamic       */
     uint32_t buckets;       /* --buckets      hi (default 512)                  */
     uint32_t bucket_size;   /* --bucket-size  hi (default 1000)                 */
     uint32_t reserved[4];   /* tuning sweeps, 0 = automatic: [0] ip SISI

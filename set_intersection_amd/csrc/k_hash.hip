@@ -13,12 +13,22 @@
 //    global memory.  Capacity is derived from |a|, so no bucket can overflow
 //    (the reference never checks bucketSize, hi.cu:329-330).  0xFFFFFFFF is
 //    the empty-slot marker; the value itself is tracked by a flag.
-//  * SISI cuts the pair into merge-path segments (A-first tie order, so a
-//    matching pair never straddles two segments once the B range includes the
-//    B element that follows it) and each workgroup hashes the SMALLER side of
-//    its segment in LDS and probes the other side.
+//  * SISI is tiled by value: the universe is cut into buckets of 2^bits
+//    values (about 4096 elements of the larger set each) and a workgroup
+//    takes a run of consecutive buckets.  Per bucket it inserts the bucket's
+//    elements of the SMALLER set into an LDS table sized for them (groups of
+//    4 slots, at least twice as many slots as elements, at least 64) and
+//    probes it with the bucket's elements of the larger set; both sets
+//    stream once from HBM in order.  A key goes to the first empty slot of
+//    its group (the next group when it is full), so a probe reads one
+//    16-byte group and compares 4 keys; at load 1/2 about 86% of the groups
+//    have an empty slot.  A slot holds the value's low `bits` bits plus one,
+//    so 0 is the empty marker and clearing is a zero fill.  With --path the
+//    pair is cut into merge-path segments instead (k_path.hip).
 //  * MISI runs keep the reference's ownership (workgroup, or one wave with
 //    --warp, per row a) and reduce each pair with wave shuffles.
+#include <cmath>
+
 #include "si_device.h"
 #include "si_internal.h"
 
@@ -64,46 +74,160 @@ __device__ __forceinline__ uint32_t log2_ceil(uint64_t v) {
 }
 
 // ---------------------------------------------------------------- SISI ----
-template <int NT, int SEG>
-__global__ __launch_bounds__(NT) void k_hash_pair(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
-                                                  uint64_t oB, uint64_t nB, uint64_t* __restrict__ partials,
-                                                  uint32_t* __restrict__ done, uint64_t* __restrict__ out64,
-                                                  uint32_t* __restrict__ out32) {
-    // the hashed side is the smaller one: <= (SEG + 1) / 2 elements, so SEG
-    // slots keep the load factor <= 1/2
-    static_assert((SEG & (SEG - 1)) == 0, "SEG must be a power of two");
-    constexpr uint32_t LOG = 31 - __builtin_clz(SEG);
-    __shared__ uint32_t tab[1u << LOG];
-    __shared__ uint64_t seg[2];
-    __shared__ uint32_t hasMax;
+// One workgroup per run of `per` buckets [v0, v1) of 2^bits values; dynamic
+// LDS holds the bucket bounds of both sets.  S (the smaller set) is hashed,
+// Q probes.  A thread holds up to U elements of each side of the current
+// bucket in registers, and the next bucket's are loaded while the current one
+// is processed.  A bucket with more than TSMAX/2 elements of S is hashed in
+// pieces of TSMAX/2, each probed by all of the bucket's Q elements; the
+// bucket width keeps the expected S elements per bucket at 3072 or fewer.
+template <int NT, int U, int TSMAX>
+__global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict__ S, uint64_t nS,
+                                                     const uint32_t* __restrict__ Q, uint64_t nQ, uint32_t bits,
+                                                     uint64_t nb, uint32_t per, uint64_t* __restrict__ partials,
+                                                     uint32_t* __restrict__ done, uint64_t* __restrict__ out64,
+                                                     uint32_t* __restrict__ out32) {
+    static_assert(TSMAX / 2 <= NT * U, "the first piece of a bucket's S elements comes from registers");
+    extern __shared__ uint64_t bnd[];  // S bounds [0, nbd), Q bounds [nbd, 2 nbd)
+    __shared__ uint4 tab4[TSMAX / 4];
     __shared__ uint64_t red[NT / WAVE];
-    const uint64_t n = nA + nB;
-    const uint64_t dBeg = uint64_t(blockIdx.x) * SEG;
-    const uint64_t dEnd = dBeg + SEG < n ? dBeg + SEG : n;
-    if (threadIdx.x < WAVE) {
-        const uint64_t c = corank_halfwave(base + oA, nA, base + oB, nB, lane_id() < 32 ? dBeg : dEnd);
-        if (lane_id() == 0) seg[0] = c;
-        if (lane_id() == 32) seg[1] = c;
-    }
-    for (uint32_t i = threadIdx.x; i < (1u << LOG); i += NT) tab[i] = EMPTY;
-    if (threadIdx.x == 0) hasMax = 0;
-    __syncthreads();
-    const uint64_t ia = seg[0], iaE = seg[1];
-    const uint64_t ib = dBeg - ia;
-    uint64_t ibE = dEnd - iaE;
-    if (ibE < nB) ++ibE;  // include the B element that follows the segment
-    const uint64_t na = iaE - ia, nb = ibE - ib;
-    // hash the smaller side
-    const bool hashA = na <= nb;
-    const uint32_t* H = base + (hashA ? oA + ia : oB + ib);
-    const uint32_t* Q = base + (hashA ? oB + ib : oA + ia);
-    const uint32_t nh = uint32_t(hashA ? na : nb), nq = uint32_t(hashA ? nb : na);
-    const uint32_t mask = (1u << LOG) - 1;
-    for (uint32_t i = threadIdx.x; i < nh; i += NT) table_insert(tab, mask, LOG, H[i], &hasMax);
-    __syncthreads();
+    uint32_t* tab = reinterpret_cast<uint32_t*>(tab4);
+    const uint32_t low = (1u << bits) - 1u;
+    const uint64_t v0 = uint64_t(blockIdx.x) * per, v1 = v0 + per < nb ? v0 + per : nb;
     uint64_t c = 0;
-    const uint32_t hm = hasMax;
-    for (uint32_t i = threadIdx.x; i < nq; i += NT) c += table_find(tab, mask, LOG, Q[i], hm);
+    if (v0 < v1) {
+        const uint32_t nbd = uint32_t(v1 - v0) + 1;
+        for (uint32_t t = threadIdx.x; t < 2 * nbd; t += NT) {
+            const bool isQ = t >= nbd;
+            const uint64_t key = (v0 + (isQ ? t - nbd : t)) << bits;  // first value of the bucket
+            const uint32_t* P = isQ ? Q : S;
+            uint64_t lo = 0, hi = isQ ? nQ : nS;  // first position with a value >= key
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (uint64_t(P[mid]) < key) lo = mid + 1; else hi = mid;
+            }
+            bnd[t] = lo;
+        }
+        __syncthreads();
+        auto work = [&](uint32_t k) {  // the next bucket with elements on both sides
+            while (k + 1 < nbd && (bnd[k] == bnd[k + 1] || bnd[nbd + k] == bnd[nbd + k + 1])) ++k;
+            return k;
+        };
+        auto fetch = [&](uint32_t k, uint32_t (&xs)[U], uint32_t (&xq)[U]) {
+            const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t j = s0 + threadIdx.x + uint64_t(u) * NT;
+                xs[u] = S[j < s1 ? j : s1 - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t j = q0 + threadIdx.x + uint64_t(u) * NT;
+                xq[u] = Q[j < q1 ? j : q1 - 1];
+            }
+        };
+        constexpr uint64_t CAP = uint64_t(NT) * U;
+        constexpr uint32_t PIECE = TSMAX / 2;
+        uint32_t xs[U], xq[U], ys[U], yq[U];
+        uint32_t k = work(0);
+        if (k + 1 < nbd) fetch(k, xs, xq);
+        while (k + 1 < nbd) {
+            const uint32_t kn = work(k + 1);
+            if (kn + 1 < nbd) fetch(kn, ys, yq);
+            const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
+            for (uint64_t p0 = s0; p0 < s1; p0 += PIECE) {
+                const uint32_t m = uint32_t(s1 - p0 < PIECE ? s1 - p0 : PIECE);
+                // 2^lg groups of 4 slots, at least 2m slots: load <= 1/2
+                uint32_t lg = 32u - uint32_t(__builtin_clz(2 * m - 1));
+                lg = (lg < 6 ? 6 : lg) - 2;
+                const uint32_t gmask = (1u << lg) - 1u;
+                for (uint32_t i = threadIdx.x; i <= gmask; i += NT) tab4[i] = make_uint4(0u, 0u, 0u, 0u);
+                __syncthreads();
+                // S elements, four at a time side by side: a round reads each
+                // pending element's group (16 bytes), claims its first empty
+                // slot with one atomic, and moves to the next group when the
+                // group is full (a lost race re-reads the same group)
+                auto insert4 = [&](const uint32_t (&x)[4], uint32_t pend) {
+                    uint32_t key[4], g[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        key[j] = (x[j] & low) + 1u;
+                        g[j] = (key[j] * 0x9E3779B1u) >> (32 - lg);
+                    }
+                    while (pend) {
+                        uint4 t[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) t[j] = ((pend >> j) & 1u) ? tab4[g[j]] : make_uint4(1u, 1u, 1u, 1u);
+                        uint32_t prev[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const uint32_t w = t[j].x == 0u ? 0u : t[j].y == 0u ? 1u : t[j].z == 0u ? 2u : t[j].w == 0u ? 3u : 4u;
+                            const bool live = (pend >> j) & 1u;
+                            // keys are at most 2^30: ~0u marks a live element not placed this round
+                            prev[j] = (live && w < 4u) ? atomicCAS(&tab[4u * g[j] + w], 0u, key[j]) : (live ? ~0u : 0u);
+                            if (live && w == 4u) g[j] = (g[j] + 1u) & gmask;
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (prev[j] == 0u || prev[j] == key[j]) pend &= ~(1u << j);
+                    }
+                };
+                if (p0 == s0) {
+#pragma unroll
+                    for (int b = 0; b < U; b += 4) {
+                        uint32_t pend = 0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) pend |= uint32_t(threadIdx.x + uint32_t(b + j) * NT < m) << j;
+                        if (pend) insert4({xs[b], xs[b + 1], xs[b + 2], xs[b + 3]}, pend);
+                    }
+                } else {
+                    for (uint64_t i = p0 + threadIdx.x; i < p0 + m; i += NT) insert4({S[i], 0u, 0u, 0u}, 1u);
+                }
+                __syncthreads();
+                // probes, four at a time: a key is present if its group or a
+                // following full group holds it; a group with an empty slot ends
+                // the search
+                auto probe4 = [&](const uint32_t (&x)[4], uint32_t pend) {
+                    uint32_t key[4], g[4], hits = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        key[j] = (x[j] & low) + 1u;
+                        g[j] = (key[j] * 0x9E3779B1u) >> (32 - lg);
+                    }
+                    while (pend) {
+                        uint4 t[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) t[j] = ((pend >> j) & 1u) ? tab4[g[j]] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const bool live = (pend >> j) & 1u;
+                            const bool hit = t[j].x == key[j] || t[j].y == key[j] || t[j].z == key[j] || t[j].w == key[j];
+                            const bool full = t[j].x && t[j].y && t[j].z && t[j].w;
+                            hits += uint32_t(live && hit);
+                            if (live && (hit || !full)) pend &= ~(1u << j);
+                            g[j] = (live && !hit && full) ? ((g[j] + 1u) & gmask) : g[j];
+                        }
+                    }
+                    return hits;
+                };
+#pragma unroll
+                for (int b = 0; b < U; b += 4) {
+                    uint32_t pend = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) pend |= uint32_t(q0 + threadIdx.x + uint64_t(b + j) * NT < q1) << j;
+                    if (pend) c += probe4({xq[b], xq[b + 1], xq[b + 2], xq[b + 3]}, pend);
+                }
+                for (uint64_t i = q0 + CAP + threadIdx.x; i < q1; i += NT) c += probe4({Q[i], 0u, 0u, 0u}, 1u);
+                __syncthreads();  // probes done before the table is cleared again
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                xs[u] = ys[u];
+                xq[u] = yq[u];
+            }
+            k = kn;
+        }
+    }
     grid_fold_u64<NT>(c, partials, done, out64, out32, red);
 }
 
@@ -141,31 +265,51 @@ __global__ __launch_bounds__(NT) void k_hash_run(const uint32_t* __restrict__ ba
 }
 
 constexpr int PAIR_NT = 256;
-constexpr int PAIR_SEG = 8192;
+constexpr int PAIR_TSMAX = 8192;  // 32 KiB table: four workgroups per CU
 
 }  // namespace
 
 namespace si_impl {
 
-int hi_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params*, uint64_t* d_out, uint32_t* d_out32) {
+int hi_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32) {
+    if (p && p->path) return path_pair_count(ds, a, b, SI_TECH_HI, p, d_out, d_out32);
     si_context* ctx = ds->ctx;
-    const uint64_t oA = ds->offsets[a], nA = ds->sizes[a], oB = ds->offsets[b], nB = ds->sizes[b];
-    if (nA == 0 || nB == 0) {
-        return zero_outputs(ctx, d_out, d_out32);
-    }
-    const uint64_t nseg = (nA + nB + PAIR_SEG - 1) / PAIR_SEG;
-    if (nseg > 0x7FFFFFFFull) return fail(SI_ERR_RANGE, "pair too large for the hash kernel grid");
+    uint32_t s = a, q = b;  // hash the smaller set, probe with the larger
+    if (ds->sizes[s] > ds->sizes[q]) std::swap(s, q);
+    const uint64_t nS = ds->sizes[s], nQ = ds->sizes[q];
+    if (nS == 0 || nQ == 0) return zero_outputs(ctx, d_out, d_out32);
+    // buckets of 2^bits values holding about 4096 elements of the larger set
+    // (what the workgroup's registers take at once)
+    const uint64_t vmax = uint64_t(std::max(ds->max_elem, ds->universe));
+    uint32_t lu = 0, ln = 0;
+    while ((uint64_t(1) << lu) <= vmax) ++lu;  // values fit in lu bits
+    while ((uint64_t(1) << (ln + 1)) <= nQ) ++ln;
+    int bits = int(lu) - int(ln) + 12;
+    // expected elements of the hashed set per bucket at most 3072, so that a
+    // bucket almost never needs a second piece of 4096 (table load <= 1/2)
+    while (bits > 6 && double(nS) * std::ldexp(1.0, bits) > 3072.0 * (double(vmax) + 1.0)) --bits;
+    bits = bits < 6 ? 6 : (bits > 30 ? 30 : bits);
+    const uint64_t nb = (vmax >> bits) + 1;
+    const uint64_t maxGrid = (p && p->blocks) ? p->blocks : uint64_t(ctx->num_cus) * 4;
+    uint64_t per = (nb + maxGrid - 1) / maxGrid;
+    const uint64_t perMax = 255;  // bucket bounds: 4 KiB of dynamic LDS beside the 32 KiB table
+    if (per > perMax) per = perMax;
+    const uint64_t g = (nb + per - 1) / per;
+    if (g > 0x7FFFFFFFull) return fail(SI_ERR_RANGE, "pair too large for the hash kernel grid");
+    const size_t lds = size_t(2) * (per + 1) * 8;
     void* ws = nullptr;
-    SI_TRY(small_workspace(ctx, 256 + nseg * 8, &ws));
+    SI_TRY(small_workspace(ctx, 256 + g * 8, &ws));
     KernelScope ks(ctx, "hi_sisi_count");
-    hipLaunchKernelGGL((k_hash_pair<PAIR_NT, PAIR_SEG>), dim3(uint32_t(nseg)), dim3(PAIR_NT), 0, ctx->stream,
-                       ds->d_elems, oA, nA, oB, nB, reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256),
+    hipLaunchKernelGGL((k_hash_buckets<PAIR_NT, 16, PAIR_TSMAX>), dim3(uint32_t(g)), dim3(PAIR_NT), lds, ctx->stream,
+                       ds->d_elems + ds->offsets[s], nS, ds->d_elems + ds->offsets[q], nQ, uint32_t(bits), nb,
+                       uint32_t(per), reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256),
                        static_cast<uint32_t*>(ws), d_out, d_out32);
     SI_HIP(hipGetLastError());
     return SI_OK;
 }
 
 int hi_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts) {
+    if (p && p->path) return path_run(ds, SI_TECH_HI, p, A, B, n, d_counts);
     si_context* ctx = ds->ctx;
     const bool self = (A.id == B.id);
     RunArgs r{A.start, A.end, A.id, B.start, B.end, B.id, n};

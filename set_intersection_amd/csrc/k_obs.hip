@@ -6,7 +6,8 @@
 // samples (phase 1, obs.cu:312-328 / 390-407), then binary-searched in global
 // memory inside that bracket (phase 2, obs.cu:331-345 / 410-424).  warpBased /
 // blockBased differ in whether a warp or a block owns a row a (obs.cu:290,
-// 369); --path applies the search inside merge-path partitions (obs.cu:438).
+// 369); --path applies the search inside merge-path partitions (obs.cu:438;
+// here k_path.hip).
 //
 // MI355X design:
 //  * Sample positions and bracket bounds use 64-bit products, so sets of any
@@ -24,37 +25,6 @@
 namespace {
 
 using namespace si_dev;
-
-// Searches x in the sorted set S (size nS) whose samples
-// cache[t] = S[t*nS/NS], t < NS, are in LDS.  Returns 1 if present.
-template <int NS>
-__device__ __forceinline__ uint32_t obs_find(const uint32_t* __restrict__ S, uint64_t nS, const uint32_t* cache,
-                                             uint32_t x) {
-    // phase 1: j = number of samples <= x
-    uint32_t lo = 0, hi = NS;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (cache[mid] <= x)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    if (lo == 0) return 0;  // x < S[0]
-    if (cache[lo - 1] == x) return 1;
-    // phase 2: S[(j-1)*nS/NS] < x < S[j*nS/NS]  (or the end of S)
-    uint64_t l = (uint64_t(lo - 1) * nS) / NS + 1;
-    uint64_t h = (lo == NS) ? nS : (uint64_t(lo) * nS) / NS;
-    while (l < h) {
-        const uint64_t mid = (l + h) >> 1;
-        const uint32_t y = S[mid];
-        if (y == x) return 1;
-        if (y < x)
-            l = mid + 1;
-        else
-            h = mid;
-    }
-    return 0;
-}
 
 template <int NT, int NS>
 __device__ __forceinline__ void load_samples(uint32_t* cache, const uint32_t* __restrict__ S, uint64_t nS) {
@@ -75,7 +45,7 @@ __global__ __launch_bounds__(NT) void k_obs_pair(const uint32_t* __restrict__ ba
     __syncthreads();
     uint64_t c = 0;
     const uint64_t stride = uint64_t(gridDim.x) * NT;
-    for (uint64_t i = uint64_t(blockIdx.x) * NT + threadIdx.x; i < nP; i += stride) c += obs_find<NS>(S, nS, cache, P[i]);
+    for (uint64_t i = uint64_t(blockIdx.x) * NT + threadIdx.x; i < nP; i += stride) c += sample_search<NS>(S, nS, cache, P[i]);
     grid_fold_u64<NT>(c, partials, done, out64, out32, red);
 }
 
@@ -94,7 +64,7 @@ __global__ __launch_bounds__(NT) void k_obs_run(const uint32_t* __restrict__ bas
         for (uint32_t b = SELF ? a + 1 : r.bStart; b < r.bEnd; ++b) {
             const uint64_t oB = offsets[b], nB = offsets[b + 1] - oB;
             uint32_t c = 0;
-            for (uint64_t i = threadIdx.x; i < nB; i += NT) c += obs_find<NT>(S, nA, cache, base[oB + i]);
+            for (uint64_t i = threadIdx.x; i < nB; i += NT) c += sample_search<NT>(S, nA, cache, base[oB + i]);
             c = wave_sum_u32(c);
             if (lane_id() == 0 && c) atomicAdd(&counts[run_index<SELF>(r, a, b)], c);
         }
@@ -110,6 +80,7 @@ constexpr int PAIR_NS = 4096;
 namespace si_impl {
 
 int obs_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32) {
+    if (p && p->path) return path_pair_count(ds, a, b, SI_TECH_OBS, p, d_out, d_out32);
     si_context* ctx = ds->ctx;
     // search the elements of the smaller set in the larger one
     uint32_t s = a, q = b;
@@ -132,6 +103,7 @@ int obs_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, u
 }
 
 int obs_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts) {
+    if (p && p->path) return path_run(ds, SI_TECH_OBS, p, A, B, n, d_counts);
     si_context* ctx = ds->ctx;
     const bool self = (A.id == B.id);
     RunArgs r{A.start, A.end, A.id, B.start, B.end, B.id, n};

@@ -182,6 +182,39 @@ __device__ __forceinline__ uint64_t corank_serial(const uint32_t* __restrict__ A
     return lo;
 }
 
+// ------------------------------------------------- sample-cache search (obs)
+// Searches x in the sorted set S (size nS) whose samples
+// cache[t] = S[t*nS/NS], t < NS, are in LDS; returns 1 if present.  Phase 1
+// brackets x between two samples in LDS, phase 2 binary-searches S inside
+// the bracket in global memory (obs.cu:312-345), with 64-bit positions.
+template <int NS>
+__device__ __forceinline__ uint32_t sample_search(const uint32_t* __restrict__ S, uint64_t nS, const uint32_t* cache,
+                                                  uint32_t x) {
+    uint32_t lo = 0, hi = NS;  // phase 1: lo = number of samples <= x
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cache[mid] <= x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo == 0) return 0;  // x < S[0]
+    if (cache[lo - 1] == x) return 1;
+    // phase 2: S[(j-1)*nS/NS] < x < S[j*nS/NS]  (or the end of S)
+    uint64_t l = (uint64_t(lo - 1) * nS) / NS + 1;
+    uint64_t h = (lo == NS) ? nS : (uint64_t(lo) * nS) / NS;
+    while (l < h) {
+        const uint64_t mid = (l + h) >> 1;
+        const uint32_t y = S[mid];
+        if (y == x) return 1;
+        if (y < x)
+            l = mid + 1;
+        else
+            h = mid;
+    }
+    return 0;
+}
+
 // ------------------------------------------------------------ pair indexing
 // triangular_index of include/util.hpp:68-70 in 64-bit arithmetic.
 __device__ __forceinline__ uint64_t tri_index(uint64_t n, uint64_t i, uint64_t j) {

@@ -146,6 +146,11 @@ int obs_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n
 // hash intersect (k_hash.hip)
 int hi_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);
 int hi_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
+// --path variants of obs and hi: merge-path segments (k_path.hip); tech is
+// SI_TECH_OBS or SI_TECH_HI
+int path_pair_count(si_dataset* ds, uint32_t a, uint32_t b, int32_t tech, const si_params* p, uint64_t* d_out,
+                    uint32_t* d_out32);
+int path_run(si_dataset* ds, int32_t tech, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
 // bitmaps (k_bitmap.hip)
 int bitmap_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);
 int bitmap_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);

@@ -21,13 +21,23 @@ SISI_TECHS = [si.TECH_IP, si.TECH_OBS, si.TECH_HI, si.TECH_BITMAP_DYNAMIC]
 RUN_TECHS = [si.TECH_IP, si.TECH_OBS, si.TECH_HI, si.TECH_BITMAP_DYNAMIC]
 
 
+PATH = si.params(path=True)
+
+
 def check_pair(ctx, a, b, techs=SISI_TECHS):
+    """Every technique in both role orders; obs and hi also with --path
+    (merge-path segments, k_path.hip) at the default geometry and with a
+    single workgroup (one range walked segment by segment)."""
     coll = si.Collection.from_sets([a, b])
     ds = ctx.upload(coll)
     exp = oracle.count(coll.set(0), coll.set(1))
     for t in techs:
         assert ds.pair_count(0, 1, t) == exp, t
         assert ds.pair_count(1, 0, t) == exp, t
+    for t in (si.TECH_OBS, si.TECH_HI):
+        for p in (PATH, si.params(path=True, blocks=1), si.params(path=True, blocks=7)):
+            assert ds.pair_count(0, 1, t, p) == exp, (t, p.blocks, "path")
+            assert ds.pair_count(1, 0, t, p) == exp, (t, p.blocks, "path")
     ds.free()
 
 
@@ -80,14 +90,27 @@ def test_sisi_skewed_2p24x2p12(ctx):
     check_pair(ctx, a, b)
 
 
-# --warp selects a kernel only for obs and hi
-@pytest.mark.parametrize("tech,warp", [(t, False) for t in RUN_TECHS] + [(si.TECH_OBS, True), (si.TECH_HI, True)])
+# --warp and --path select kernels only for obs and hi
+@pytest.mark.parametrize("tech,warp,path", [(t, False, False) for t in RUN_TECHS] +
+                         [(t, w, p) for t in (si.TECH_OBS, si.TECH_HI) for w, p in ((True, False), (False, True))])
 @pytest.mark.parametrize("name,partition", [("closed_form", 10000), ("closed_form", 3), ("random_k12", 5),
                                             ("random_k23", 10000), ("random_k23", 4)])
-def test_golden_runs(ctx, tech, warp, name, partition):
+def test_golden_runs(ctx, tech, warp, path, name, partition):
     coll = datasets.read_bin(os.path.join(GOLDEN, name + ".bin"))
     ds = ctx.upload(coll)
-    got = run_layout_counts(ds, coll, tech, partition, si.params(warp=warp))
+    got = run_layout_counts(ds, coll, tech, partition, si.params(warp=warp, path=path))
+    check_all_pairs(got, coll)
+
+
+@pytest.mark.parametrize("tech", [si.TECH_OBS, si.TECH_HI])
+def test_path_runs_long_pairs(ctx, tech):
+    """Runs with --path on pairs longer than one segment (hash: 2048 merged
+    elements per segment, obs: 512), across tiles (partition 40 of 90 sets),
+    with dense value ranges so that matches fall on segment seams."""
+    sizes = [70000, 9000, 4097, 2049, 2048, 513, 100, 3, 1, 0] * 9
+    coll = datasets.sized_collection(sizes, 1 << 17, 12)
+    ds = ctx.upload(coll)
+    got = run_layout_counts(ds, coll, tech, 40, si.params(path=True))
     check_all_pairs(got, coll)
 
 
