@@ -204,6 +204,21 @@ int si_pair_count(si_dataset* ds, uint32_t a, uint32_t b, int32_t technique,
 int si_pair_count_async(si_dataset* ds, uint32_t a, uint32_t b, int32_t technique,
                         const si_params* params, si_buffer* out, uint64_t offset);
 
+/* Allocation-free form of si_pair_count_async: same arguments, techniques
+ * and result.  No counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The count kernels' scratch (one uint64
+ * per workgroup plus a completion ticket) comes from a file-scope device
+ * array of SI_NOALLOC_SCRATCH_BYTES per GPU, set up by si_open, instead of
+ * library-allocated memory; no allocator is called.  A launch that would need
+ * more scratch (more than 8160 workgroups, e.g. from a large
+ * si_params.blocks) fails with SI_ERR_RANGE.  All calls on one GPU share the
+ * array: calls that overlap in time on one GPU (other contexts, other host
+ * threads) race and must be serialised by the caller. */
+#define SI_NOALLOC_SCRATCH_BYTES (64u * 1024u)
+int si_pair_count_noalloc(si_dataset* ds, uint32_t a, uint32_t b, int32_t technique,
+                          const si_params* params, si_buffer* out, uint64_t offset);
+
 /* Materialises S_a ∩ S_b in ascending order (the std::set_intersection output
  * vector of std_set_intersection.cpp:68-72).  `cap` is the capacity of `out`
  * in elements; *n receives the intersection size.  SI_ERR_CAPACITY (with *n

@@ -252,6 +252,11 @@ class Dataset:
     def pair_count_async(self, a, b, out, offset=0, technique=TECH_IP, p=None):
         _check(_abi.lib().si_pair_count_async(self.h, a, b, technique, byref(p) if p else None, out.h, offset))
 
+    def pair_count_noalloc(self, a, b, out, offset=0, technique=TECH_IP, p=None):
+        """si_pair_count_noalloc: as pair_count_async, with the kernels'
+        scratch in the library's static device array (no allocation)."""
+        _check(_abi.lib().si_pair_count_noalloc(self.h, a, b, technique, byref(p) if p else None, out.h, offset))
+
     def pair_materialize(self, a, b):
         cap = int(min(self.coll.sizes[a], self.coll.sizes[b]))
         out = np.empty(max(cap, 1), dtype=np.uint32)

@@ -83,6 +83,7 @@ SIGNATURES = {
     "si_dataset_upload_events": (c_int, [P, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "si_pair_count": (c_int, [P, c_uint32, c_uint32, c_int32, POINTER(si_params), U64P]),
     "si_pair_count_async": (c_int, [P, c_uint32, c_uint32, c_int32, POINTER(si_params), P, c_uint64]),
+    "si_pair_count_noalloc": (c_int, [P, c_uint32, c_uint32, c_int32, POINTER(si_params), P, c_uint64]),
     "si_pair_materialize": (c_int, [P, c_uint32, c_uint32, U32P, c_uint64, U64P]),
     "si_pair_materialize_async": (c_int, [P, c_uint32, c_uint32, P, P]),
     "si_run_count": (c_int, [P, c_int32, POINTER(si_params), si_tile, si_tile, c_uint32, U32P]),

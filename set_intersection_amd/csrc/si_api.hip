@@ -32,13 +32,10 @@ int check_run(si_dataset* ds, si_tile A, si_tile B, uint32_t n) {
 
 }  // namespace
 
-extern "C" {
+namespace si_impl {
 
-int si_pair_count_async(si_dataset* ds, uint32_t a, uint32_t b, int32_t technique, const si_params* params,
-                        si_buffer* out, uint64_t offset) {
-    SI_TRY(check_pair(ds, a, b));
-    if (!out || offset + 8 > out->bytes || (offset & 7)) return fail(SI_ERR_ARG, "bad output buffer/offset");
-    uint64_t* d = reinterpret_cast<uint64_t*>(static_cast<char*>(out->ptr) + offset);
+int pair_count_dispatch(si_dataset* ds, uint32_t a, uint32_t b, int32_t technique, const si_params* params,
+                        uint64_t* d) {
     switch (technique) {
         case SI_TECH_IP: return ip_pair_count(ds, a, b, params, d, nullptr);
         case SI_TECH_OBS: return obs_pair_count(ds, a, b, params, d, nullptr);
@@ -46,6 +43,18 @@ int si_pair_count_async(si_dataset* ds, uint32_t a, uint32_t b, int32_t techniqu
         case SI_TECH_BITMAP_DYNAMIC: return bitmap_pair_count(ds, a, b, params, d, nullptr);
         default: return not_available("si_pair_count");
     }
+}
+
+}  // namespace si_impl
+
+extern "C" {
+
+int si_pair_count_async(si_dataset* ds, uint32_t a, uint32_t b, int32_t technique, const si_params* params,
+                        si_buffer* out, uint64_t offset) {
+    SI_TRY(check_pair(ds, a, b));
+    if (!out || offset + 8 > out->bytes || (offset & 7)) return fail(SI_ERR_ARG, "bad output buffer/offset");
+    uint64_t* d = reinterpret_cast<uint64_t*>(static_cast<char*>(out->ptr) + offset);
+    return pair_count_dispatch(ds, a, b, technique, params, d);
 }
 
 int si_pair_count(si_dataset* ds, uint32_t a, uint32_t b, int32_t technique, const si_params* params, uint64_t* count) {

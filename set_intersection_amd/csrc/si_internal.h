@@ -83,6 +83,10 @@ struct si_context {
     si_impl::DevMem ws_aux2;
     bool ws_small_zeroed = false;
     int sisi_shape = 0;  // tile shape of the SISI merge kernel used by materialisation
+    // si_pair_count_noalloc: this device's copy of the file-scope scratch
+    // array, and whether the current call must take its workspace from it
+    void* noalloc_ws = nullptr;
+    bool in_noalloc = false;
     // objects created from this context; si_close frees whatever is left
     std::set<si_buffer*> buffers;
     std::set<si_dataset*> datasets;
@@ -126,8 +130,17 @@ struct KernelScope {
 
 int check_pair(si_dataset* ds, uint32_t a, uint32_t b);
 
+// Resolves and clears this device's copy of the allocation-free scratch
+// array (si_noalloc.hip); called by si_open.
+int noalloc_setup(si_context* ctx);
+
+// The technique switch of si_pair_count_async: count of (a, b) into d_out.
+int pair_count_dispatch(si_dataset* ds, uint32_t a, uint32_t b, int32_t technique, const si_params* params,
+                        uint64_t* d_out);
+
 // Zero-initialised per-context counters region of `bytes` bytes.  Kernels
-// that use it leave it zeroed again when they finish.
+// that use it leave it zeroed again when they finish.  Inside
+// si_pair_count_noalloc it is the scratch array (SI_ERR_RANGE beyond it).
 int small_workspace(si_context* ctx, uint64_t bytes, void** out);
 
 // Writes 0 to whichever of the two result slots is given (a pair with an

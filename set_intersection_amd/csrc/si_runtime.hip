@@ -68,7 +68,7 @@ static void drain_pending(si_context* ctx) {
 KernelScope::KernelScope(si_context* c, const char* n) : KernelScope(c, n, c->stream) {}
 
 KernelScope::KernelScope(si_context* c, const char* n, hipStream_t s) : ctx(c), name(n), stream(s) {
-    if (!ctx->kernel_timing) return;
+    if (!ctx->kernel_timing || ctx->in_noalloc) return;  // the allocation-free entry point creates no events
     start = take_event(ctx);
     end = take_event(ctx);
     if (start) (void)hipEventRecord(start, stream);
@@ -89,6 +89,14 @@ int check_pair(si_dataset* ds, uint32_t a, uint32_t b) {
 }
 
 int small_workspace(si_context* ctx, uint64_t bytes, void** out) {
+    if (ctx->in_noalloc) {
+        if (bytes > SI_NOALLOC_SCRATCH_BYTES)
+            return fail(SI_ERR_RANGE, "the launch needs " + std::to_string(bytes) +
+                                          " B of scratch; the allocation-free entry point has " +
+                                          std::to_string(SI_NOALLOC_SCRATCH_BYTES) + " B (lower si_params.blocks)");
+        *out = ctx->noalloc_ws;
+        return SI_OK;
+    }
     if (bytes > ctx->ws_small.bytes || !ctx->ws_small.ptr) {
         uint64_t want = std::max<uint64_t>(bytes, 1 << 16);
         SI_TRY(ctx->ws_small.ensure(want));
@@ -102,9 +110,16 @@ int small_workspace(si_context* ctx, uint64_t bytes, void** out) {
     return SI_OK;
 }
 
+// (a one-lane kernel rather than hipMemsetAsync: memset nodes have given
+// wrong results on graph replays)
+__global__ void k_zero_outputs(uint64_t* __restrict__ out64, uint32_t* __restrict__ out32) {
+    if (out64) *out64 = 0;
+    if (out32) *out32 = 0;
+}
+
 int zero_outputs(si_context* ctx, uint64_t* d_out64, uint32_t* d_out32) {
-    if (d_out64) SI_HIP(hipMemsetAsync(d_out64, 0, 8, ctx->stream));
-    if (d_out32) SI_HIP(hipMemsetAsync(d_out32, 0, 4, ctx->stream));
+    hipLaunchKernelGGL(k_zero_outputs, dim3(1), dim3(1), 0, ctx->stream, d_out64, d_out32);
+    SI_HIP(hipGetLastError());
     return SI_OK;
 }
 
@@ -198,6 +213,11 @@ int si_open(int device, si_context** out) {
         return fail(SI_ERR_HIP, std::string("si_open: ") + hipGetErrorString(e));
     }
     ctx->num_cus = ctx->prop.multiProcessorCount;
+    if (noalloc_setup(ctx) != SI_OK) {
+        const std::string msg = si_last_error();
+        si_close(ctx);
+        return fail(SI_ERR_HIP, "si_open: " + msg);
+    }
     // tile shape of the SISI merge kernel for materialisation (tuning sweeps)
     if (const char* e = std::getenv("SI_SISI_SHAPE")) ctx->sisi_shape = std::atoi(e);
     *out = ctx;
