@@ -226,7 +226,10 @@ int si_pair_count_noalloc(si_dataset* ds, uint32_t a, uint32_t b, int32_t techni
 int si_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* out,
                         uint64_t cap, uint64_t* n);
 /* Device form: writes the list to `out` (device) and the size as uint64 at
- * byte offset 0 of `count_out`.  `out` must hold min(|S_a|,|S_b|) elements. */
+ * byte offset 0 of `count_out`.  `out` must hold min(|S_a|,|S_b|) elements.
+ * A dataset uploaded with validate = 0 is validated (once, synchronously) by
+ * its first materialisation: SI_ERR_UNSORTED if a set is not strictly
+ * ascending. */
 int si_pair_materialize_async(si_dataset* ds, uint32_t a, uint32_t b, si_buffer* out,
                               si_buffer* count_out);
 

@@ -77,6 +77,10 @@ int si_pair_materialize_async(si_dataset* ds, uint32_t a, uint32_t b, si_buffer*
     if (!out || !count_out || count_out->bytes < 8) return fail(SI_ERR_ARG, "bad output buffers");
     const uint64_t need = std::min(ds->sizes[a], ds->sizes[b]);
     if (out->bytes < need * 4) return fail(SI_ERR_CAPACITY, "output buffer smaller than min(|a|,|b|) elements");
+    // the staging slots and the output bound hold for strictly ascending sets
+    // only (a run of duplicates would match more than min(|a|,|b|) times): a
+    // dataset uploaded without validation is checked once, here
+    if (!ds->validated) SI_TRY(validate_dataset(ds));
     return ip_pair_materialize(ds, a, b, static_cast<uint32_t*>(out->ptr), static_cast<uint64_t*>(count_out->ptr));
 }
 

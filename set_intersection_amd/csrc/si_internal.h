@@ -113,6 +113,7 @@ struct si_dataset {
     uint32_t max_elem = 0;          // largest element of the collection
     si_impl::BatchPlan* plan = nullptr;  // batched dispatcher state (built on first use)
     int32_t upload_events[3] = {-1, -1, -1};  // context event ids: transfers, offsets scan, end
+    bool validated = false;         // every set checked strictly ascending
 };
 
 namespace si_impl {
@@ -129,6 +130,10 @@ struct KernelScope {
 };
 
 int check_pair(si_dataset* ds, uint32_t a, uint32_t b);
+
+// Device check that every set of ds is strictly ascending; sets
+// ds->validated (SI_ERR_UNSORTED if a set is not).
+int validate_dataset(si_dataset* ds);
 
 // Resolves and clears this device's copy of the allocation-free scratch
 // array (si_noalloc.hip); called by si_open.

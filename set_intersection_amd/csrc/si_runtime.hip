@@ -501,20 +501,9 @@ int si_dataset_upload(si_context* ctx, uint32_t k, uint32_t universe, uint64_t t
     // the largest element (sets are sorted: the max of the last elements)
     for (uint32_t i = 0; i < k; ++i)
         if (sizes[i]) ds->max_elem = std::max(ds->max_elem, elements[ds->offsets[i + 1] - 1]);
-    if (validate && total > 1) {
-        void* ws = nullptr;
-        int st = small_workspace(ctx, 256, &ws);
+    if (validate) {
+        const int st = validate_dataset(ds);
         if (st != SI_OK) return cleanup(st, si_last_error());
-        uint32_t* bad = static_cast<uint32_t*>(ws) + 32;  // slot reserved for validation
-        uint32_t grid = uint32_t(std::min<uint64_t>((total + 255) / 256, uint64_t(ctx->num_cus) * 16));
-        hipLaunchKernelGGL(k_check_sorted, dim3(grid), dim3(256), 0, ctx->stream, ds->d_elems, ds->d_offsets, k, total,
-                           bad);
-        uint32_t nbad = 0;
-        e = hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(bad, 0, 4, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("validation: ") + hipGetErrorString(e));
-        if (nbad) return cleanup(SI_ERR_UNSORTED, std::to_string(nbad) + " elements are not strictly ascending within their set");
     }
     e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return cleanup(SI_ERR_HIP, std::string("dataset upload: ") + hipGetErrorString(e));
@@ -522,6 +511,36 @@ int si_dataset_upload(si_context* ctx, uint32_t k, uint32_t universe, uint64_t t
     *out = ds;
     return SI_OK;
 }
+
+}  // extern "C"
+
+namespace si_impl {
+
+// Checks on the device that every set is strictly ascending
+// (SI_ERR_UNSORTED otherwise) and marks the dataset validated.
+int validate_dataset(si_dataset* ds) {
+    si_context* ctx = ds->ctx;
+    if (ds->total > 1) {
+        void* ws = nullptr;
+        SI_TRY(small_workspace(ctx, 256, &ws));
+        uint32_t* bad = static_cast<uint32_t*>(ws) + 32;  // slot reserved for validation
+        const uint32_t grid = uint32_t(std::min<uint64_t>((ds->total + 255) / 256, uint64_t(ctx->num_cus) * 16));
+        hipLaunchKernelGGL(k_check_sorted, dim3(grid), dim3(256), 0, ctx->stream, ds->d_elems, ds->d_offsets, ds->k,
+                           ds->total, bad);
+        uint32_t nbad = 0;
+        SI_HIP(hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
+        SI_HIP(hipMemsetAsync(bad, 0, 4, ctx->stream));
+        SI_HIP(hipStreamSynchronize(ctx->stream));
+        if (nbad)
+            return fail(SI_ERR_UNSORTED, std::to_string(nbad) + " elements are not strictly ascending within their set");
+    }
+    ds->validated = true;
+    return SI_OK;
+}
+
+}  // namespace si_impl
+
+extern "C" {
 
 int si_dataset_free(si_dataset* ds) {
     if (!ds) return SI_OK;

@@ -158,6 +158,21 @@ def test_errors(ctx):
         ctx.upload(si.Collection.from_sets([[1, 1, 2]]))
 
 
+def test_materialise_unvalidated(ctx):
+    """A dataset uploaded without validation is checked by its first
+    materialisation: a run of duplicates (which would match more often than
+    min(|a|, |b|)) is refused with SI_ERR_UNSORTED; valid sets materialise."""
+    dup = si.Collection.from_sets([np.repeat(np.arange(0, 5000, dtype=np.uint32), 3),
+                                   np.repeat(np.arange(0, 5000, dtype=np.uint32), 2)])
+    ds = ctx.upload(dup, validate=False)
+    with pytest.raises(si.SIError) as e:
+        ds.pair_materialize(0, 1)
+    assert e.value.status == -5
+    good = datasets.sisi_pair(100_000, 80_000, 1 << 20, 3)
+    ds2 = ctx.upload(good, validate=False)
+    assert np.array_equal(ds2.pair_materialize(0, 1), oracle.materialize(good.set(0), good.set(1)))
+
+
 # ------------------------------------------------------------------ MISI --
 @pytest.mark.parametrize("name", ["closed_form", "random_k12", "random_k23"])
 @pytest.mark.parametrize("partition", [3, 5, 10000])
