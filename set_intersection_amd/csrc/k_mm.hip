@@ -75,7 +75,23 @@ __device__ __forceinline__ v4i expand_b4(uint32_t x) {
 typedef int v8i __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ v8i f4_operand(const v4i& f) { return v8i{f[0], f[1], f[2], f[3], 0, 0, 0, 0}; }
 
-enum { MODE_I8 = 0, MODE_F8 = 1, MODE_F4 = 2 };
+enum { MODE_I8 = 0, MODE_F8 = 1, MODE_F4 = 2, MODE_I8W = 3 };
+
+// Weighted int8 operands (MODE_I8W): the same bits as expand32, but A's dword
+// j carries a set bit as the byte value 2^j (a mask in place, no shift) and
+// B's as 2^(3-j) (bit j moved to bit 3-j), so the product of two set bits is
+// 8 in every dword and the count is the accumulated sum / 8.  A fragment is
+// expanded twice as often as a B fragment (8 against 4 per K step), so the
+// shift-free side is A.  Exact while every count is below 2^28.
+__device__ __forceinline__ v4i expand_w_a(uint32_t x, uint32_t p) {
+    const uint32_t y = x >> (4 * p);
+    return v4i{int(y & 0x01010101u), int(y & 0x02020202u), int(y & 0x04040404u), int(y & 0x08080808u)};
+}
+__device__ __forceinline__ v4i expand_w_b(uint32_t x, uint32_t p) {
+    const uint32_t y = x >> (4 * p);
+    return v4i{int((y << 3) & 0x08080808u), int((y << 1) & 0x04040404u), int((y >> 1) & 0x02020202u),
+               int((y >> 3) & 0x01010101u)};
+}
 
 template <bool F8>
 __device__ __forceinline__ v4i expand32(uint32_t x, uint32_t p) {
@@ -165,7 +181,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
 
-    using Acc = typename std::conditional<MODE != MODE_I8, v4f, v4i>::type;
+    constexpr bool INT = MODE == MODE_I8 || MODE == MODE_I8W;
+    using Acc = typename std::conditional<INT, v4i, v4f>::type;
     Acc acc[8][4];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
@@ -225,7 +242,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
                 for (int j = 0; j < 4; ++j) fb[j] = fbn[j];
             }
-        } else if constexpr (MODE == MODE_I8) {
+        } else if constexpr (INT) {
             // software pipeline over the 8 A fragments of a K step: while the
             // 4 MFMAs of fragment i issue, fragment i+1 (and from i = 3 on one
             // B fragment of the next K step) is expanded, so the expansion
@@ -237,8 +254,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             for (int j = 0; j < 4; ++j) wb[j] = dB[rowB + j * 16 * RS];
             v4i fb[4], fbn[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) fb[j] = expand32<false>(wb[j], p);
-            v4i fan = expand32<false>(wa[0], p);
+            for (int j = 0; j < 4; ++j) fb[j] = MODE == MODE_I8W ? expand_w_b(wb[j], p) : expand32<false>(wb[j], p);
+            v4i fan = MODE == MODE_I8W ? expand_w_a(wa[0], p) : expand32<false>(wa[0], p);
 #pragma unroll 2
             for (int kk = 0; kk < KC; ++kk) {
                 const bool more = kk + 1 < KC;
@@ -251,9 +268,10 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const v4i fa = fan;
-                    if (i < 7) fan = expand32<false>(wa[i + 1], p);
-                    else fan = expand32<false>(na[0], p);
-                    if (i >= 3 && i < 7) fbn[i - 3] = expand32<false>(nb[i - 3], p);
+                    const uint32_t wn1 = i < 7 ? wa[i + 1] : na[0];
+                    fan = MODE == MODE_I8W ? expand_w_a(wn1, p) : expand32<false>(wn1, p);
+                    if (i >= 3 && i < 7)
+                        fbn[i - 3] = MODE == MODE_I8W ? expand_w_b(nb[i - 3], p) : expand32<false>(nb[i - 3], p);
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[j], acc[i][j], 0, 0, 0);
@@ -298,7 +316,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             for (int v = 0; v < 4; ++v) {
                 const uint32_t row = tm * TILE + i * 16 + 4 * (lane >> 4) + v;
                 const uint32_t col = tn * TILE + wn * 64 + j * 16 + (lane & 15);
-                const uint32_t val = uint32_t(acc[i][j][v]);
+                const uint32_t val = MODE == MODE_I8W ? uint32_t(acc[i][j][v]) >> 3 : uint32_t(acc[i][j][v]);
                 if (row < ni && col < nj) {
                     if (split) {
                         atomicAdd(&C[uint64_t(row) * ld + col], val);
@@ -392,6 +410,8 @@ int mm_tile_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint64
         hipLaunchKernelGGL((k_mm_bits<MODE_F8>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
     else if (variant == SI_MM_FP4)
         hipLaunchKernelGGL((k_mm_bits<MODE_F4>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
+    else if (ds->max_size < (1u << 28))  // weighted operands: 8 x count below 2^31
+        hipLaunchKernelGGL((k_mm_bits<MODE_I8W>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
     else
         hipLaunchKernelGGL((k_mm_bits<MODE_I8>), dim3(units), dim3(128), 0, ctx->stream, bA, ni, bB, nj, w64, g, d_C, ld);
     SI_HIP(hipGetLastError());
