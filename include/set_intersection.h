@@ -305,6 +305,12 @@ int si_batched_count_async(si_dataset* ds, const si_tuning* tuning, si_buffer* c
 /* Times candidate settings on this dataset (each candidate run once, until
  * budget_ms of measured time is spent) and returns the fastest record. */
 int si_batched_autotune(si_dataset* ds, double budget_ms, si_tuning* out);
+/* As si_batched_autotune, starting the search from `seed` (a record stored
+ * for a dataset of similar shape, e.g. the nearest one by size and
+ * element-degree statistics) with candidate values around the seed's
+ * (seed / 4 .. seed * 4 per knob) instead of the fixed lists.
+ * SI_ERR_ARG if the seed is not a valid record. */
+int si_batched_autotune_seeded(si_dataset* ds, const si_tuning* seed, double budget_ms, si_tuning* out);
 
 /* ------------------------------------------------------ edge-wise counts -- */
 /* For every edge e = (u[e], v[e]) of a graph whose adjacency lists are the

@@ -306,9 +306,13 @@ class Dataset:
     def batched_count_async(self, counts, tuning=None):
         _check(_abi.lib().si_batched_count_async(self.h, byref(tuning) if tuning else None, counts.h))
 
-    def autotune(self, budget_ms=2000.0):
+    def autotune(self, budget_ms=2000.0, seed=None):
+        """si_batched_autotune, or si_batched_autotune_seeded from `seed`."""
         t = si_tuning()
-        _check(_abi.lib().si_batched_autotune(self.h, budget_ms, byref(t)))
+        if seed is None:
+            _check(_abi.lib().si_batched_autotune(self.h, budget_ms, byref(t)))
+        else:
+            _check(_abi.lib().si_batched_autotune_seeded(self.h, byref(seed), budget_ms, byref(t)))
         return t
 
     def edge_count(self, u, v, tuning=None):

@@ -96,6 +96,7 @@ SIGNATURES = {
     "si_batched_count": (c_int, [P, POINTER(si_tuning), U32P]),
     "si_batched_count_async": (c_int, [P, POINTER(si_tuning), P]),
     "si_batched_autotune": (c_int, [P, c_double, POINTER(si_tuning)]),
+    "si_batched_autotune_seeded": (c_int, [P, POINTER(si_tuning), c_double, POINTER(si_tuning)]),
     "si_edge_plan_create": (c_int, [P, U32P, U32P, c_uint64, POINTER(si_tuning), POINTER(P)]),
     "si_edge_plan_free": (c_int, [P]),
     "si_edge_count_async": (c_int, [P, P, P]),

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "format.hpp"
+#include "tune_store.hpp"
 
 template <typename A, typename B>
 static bool writePairs(const std::vector<std::pair<A, B>>& v, const std::string& path) {
@@ -69,6 +70,11 @@ int main(int argc, char** argv) {
         std::printf("write error\n");
         return 1;
     }
+    // the summary the batched dispatcher's tuning store keys its records on
+    const si::ShapeFeatures f = si::features(d);
+    std::printf("set size q50 %u q90 %u q99 %u max %u\n", f.q50, f.q90, f.q99, f.maxSize);
+    std::printf("distinct elements %llu  element degree mean %.6g  second moment %.6g\n",
+                (unsigned long long)f.distinct, f.degMean, f.deg2);
     std::printf("Finished\n");
     return 0;
 }

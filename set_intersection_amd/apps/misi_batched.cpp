@@ -5,8 +5,11 @@
 // file: single-run order when k <= partition, else the multi-run order of
 // io.hpp:104-136), with the technique chosen per size bin.
 //   --tune            time candidate settings on this dataset first
-//   --tune-file P     JSON store of tuning records keyed by dataset shape:
-//                     read before the run, updated after --tune
+//   --tune-file P     JSON store of tuning records with the dataset's shape
+//                     features (set-size and element-degree statistics,
+//                     host/tune_store.hpp): the record of the same shape, or
+//                     else of the nearest one, is used (and seeds --tune);
+//                     --tune writes this dataset's record back
 //   --no-graph        launch the kernels directly instead of a hipGraph replay
 //   --repeat N        run the count N times (reports the best time)
 #include <algorithm>
@@ -22,6 +25,7 @@
 #include "set_intersection.h"
 #include "tiles.hpp"
 #include "timers.hpp"
+#include "tune_store.hpp"
 
 namespace {
 
@@ -29,69 +33,6 @@ int die(si_context* ctx) {
     std::fprintf(stderr, "GPUassert: %s\n", si_last_error());
     if (ctx) si_close(ctx);
     return 1;
-}
-
-// Shape key: cardinality, universe bits, total and size quantiles.
-std::string shapeKey(const si::Dataset& d) {
-    std::vector<uint32_t> s(d.sizes);
-    std::sort(s.begin(), s.end());
-    auto q = [&](double f) { return s.empty() ? 0u : s[std::min<size_t>(s.size() - 1, size_t(f * s.size()))]; };
-    uint32_t ubits = 0;
-    while (ubits < 32 && (uint64_t(1) << ubits) <= d.universe) ++ubits;
-    std::ostringstream o;
-    o << "k=" << d.cardinality << ";ubits=" << ubits << ";total=" << d.totalElements << ";q50=" << q(0.5)
-      << ";q90=" << q(0.9) << ";q99=" << q(0.99) << ";max=" << (s.empty() ? 0 : s.back());
-    return o.str();
-}
-
-const char* kFields[] = {"bitmap_min", "ratio_search", "chunk_elems", "chunk_elems_bitmap", "probe_wg_per_cu",
-                         "use_graph"};
-
-uint32_t* field(si_tuning& t, int i) {
-    uint32_t* f[] = {&t.bitmap_min, &t.ratio_search, &t.chunk_elems, &t.chunk_elems_bitmap, &t.probe_wg_per_cu,
-                     &t.use_graph};
-    return f[i];
-}
-
-// The store is one JSON object {"<shape>": {"field": value, ...}, ...}.
-std::map<std::string, si_tuning> loadStore(const std::string& path) {
-    std::map<std::string, si_tuning> m;
-    std::ifstream f(path);
-    if (!f) return m;
-    std::stringstream ss;
-    ss << f.rdbuf();
-    const std::string s = ss.str();
-    size_t p = 0;
-    while ((p = s.find("\"k=", p)) != std::string::npos) {
-        const size_t e = s.find('"', p + 1);
-        const std::string key = s.substr(p + 1, e - p - 1);
-        const size_t ob = s.find('{', e), cb = s.find('}', ob);
-        if (ob == std::string::npos || cb == std::string::npos) break;
-        const std::string body = s.substr(ob, cb - ob);
-        si_tuning t{};
-        t.version = SI_TUNING_VERSION;
-        for (int i = 0; i < 6; ++i) {
-            const std::string name = std::string("\"") + kFields[i] + "\"";
-            const size_t fp = body.find(name);
-            if (fp != std::string::npos) *field(t, i) = uint32_t(std::stoul(body.substr(body.find(':', fp) + 1)));
-        }
-        m[key] = t;
-        p = cb;
-    }
-    return m;
-}
-
-void saveStore(const std::string& path, const std::map<std::string, si_tuning>& m) {
-    std::ofstream f(path);
-    f << "{\n";
-    size_t n = 0;
-    for (auto& kv : m) {
-        f << "  \"" << kv.first << "\": {";
-        si_tuning t = kv.second;
-        for (int i = 0; i < 6; ++i) f << (i ? ", " : "") << "\"" << kFields[i] << "\": " << *field(t, i);
-        f << "}" << (++n < m.size() ? "," : "") << "\n";
-    }
-    f << "}\n";
 }
 
 }  // namespace
@@ -147,33 +88,45 @@ int main(int argc, char** argv) {
         return die(ctx);
     deviceTimer.addUpload(ds);
 
-    const std::string key = shapeKey(d);
     si_tuning t;
     si_batched_default_tuning(ds, &t);
-    std::map<std::string, si_tuning> store;
-    bool fromStore = false;
+    std::string source = "default";
+    si::ShapeFeatures shape;
+    std::vector<si::StoreEntry> store;
+    const si::StoreEntry* seed = nullptr;
+    if (tune || !tuneFile.empty()) {
+        shape = si::features(d);
+        si::printTable("Shape features", {{"Set size q50/q90/q99", si::toText(shape.q50) + "/" + si::toText(shape.q90) +
+                                                                       "/" + si::toText(shape.q99)},
+                                          {"Set size max", si::toText(shape.maxSize)},
+                                          {"Distinct elements", si::toText(shape.distinct)},
+                                          {"Element degree mean", si::toText(shape.degMean)},
+                                          {"Element degree 2nd moment", si::toText(shape.deg2)}});
+    }
     if (!tuneFile.empty()) {
-        store = loadStore(tuneFile);
-        auto it = store.find(key);
-        if (it != store.end()) {
-            t = it->second;
-            fromStore = true;
+        store = si::loadStore(tuneFile);
+        double dist = 0.0;
+        seed = si::nearestEntry(store, shape, 4.0, &dist);
+        if (seed) {
+            t = seed->t;
+            source = dist == 0.0 ? "stored" : "nearest stored (distance " + si::toText(dist) + ")";
         }
     }
     if (tune) {
         si::HostTimer ht;
         si::Interval* iv = ht.add("Autotune");
-        if (si_batched_autotune(ds, 20000.0, &t) != SI_OK) return die(ctx);
+        const int st = seed ? si_batched_autotune_seeded(ds, &seed->t, 20000.0, &t) : si_batched_autotune(ds, 20000.0, &t);
+        if (st != SI_OK) return die(ctx);
         si::HostTimer::finish(iv);
         ht.print();
+        source = seed ? "autotuned from the stored record" : "autotuned";
         if (!tuneFile.empty()) {
-            store[key] = t;
-            saveStore(tuneFile, store);
+            si::putEntry(store, si::StoreEntry{shape, t});
+            if (!si::saveStore(tuneFile, store)) std::printf("cannot write %s\n", tuneFile.c_str());
         }
     }
     if (noGraph) t.use_graph = 0;
-    si::printTable("Dispatcher", {{"Shape", key.substr(0, 25)},
-                                  {"Tuning", tune ? "autotuned" : fromStore ? "stored" : "default"},
+    si::printTable("Dispatcher", {{"Tuning", source},
                                   {"bitmap_min", si::toText(t.bitmap_min)},
                                   {"ratio_search", si::toText(t.ratio_search)},
                                   {"chunk_elems", si::toText(t.chunk_elems)},

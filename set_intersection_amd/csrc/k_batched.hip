@@ -982,10 +982,28 @@ int batched_count(si_dataset* ds, const si_tuning* tuning, uint32_t* d_out) {
     return SI_OK;
 }
 
-int batched_autotune(si_dataset* ds, double budget_ms, si_tuning* best) {
+// Candidate values of a knob around a seed value: seed / 4 .. seed * 4 in
+// factors of two, clamped to [lo, hi].
+static std::vector<uint32_t> around(uint32_t seed, uint32_t lo, uint32_t hi) {
+    std::vector<uint32_t> v;
+    for (int s = -2; s <= 2; ++s) {
+        uint64_t x = s < 0 ? (uint64_t(seed) >> -s) : (uint64_t(seed) << s);
+        x = std::max<uint64_t>(lo, std::min<uint64_t>(hi, x));
+        if (std::find(v.begin(), v.end(), uint32_t(x)) == v.end()) v.push_back(uint32_t(x));
+    }
+    return v;
+}
+
+int batched_autotune(si_dataset* ds, double budget_ms, si_tuning* best) { return batched_autotune_seeded(ds, nullptr, budget_ms, best); }
+
+int batched_autotune_seeded(si_dataset* ds, const si_tuning* seed, double budget_ms, si_tuning* best) {
     si_context* ctx = ds->ctx;
     si_tuning base;
     batched_default_tuning(ds, &base);
+    if (seed) {
+        SI_TRY(check_tuning(*seed));
+        base = *seed;
+    }
     base.use_graph = 0;
     *best = base;
     if (ds->k < 2) return SI_OK;
@@ -1012,14 +1030,23 @@ int batched_autotune(si_dataset* ds, double budget_ms, si_tuning* best) {
     double ms = 0;
     st = measure(base, ms);
     if (st == SI_OK) bestMs = ms;
-    // coordinate descent over the dispatcher's knobs
-    const std::vector<std::pair<uint32_t si_tuning::*, std::vector<uint32_t>>> knobs = {
+    // coordinate descent over the dispatcher's knobs: fixed candidate lists,
+    // or values around the seed's when the descent starts from a seed
+    std::vector<std::pair<uint32_t si_tuning::*, std::vector<uint32_t>>> knobs = {
         {&si_tuning::bitmap_min, {512, 1024, 2048, 4096}},
         {&si_tuning::chunk_elems, {16384, 32768, 65536, 131072, 262144}},
         {&si_tuning::chunk_elems_bitmap, {262144, 524288, 1u << 20, 1u << 21}},
         {&si_tuning::probe_wg_per_cu, {2, 4, 6, 8}},
         {&si_tuning::ratio_search, {8, 16, 32, 64}},
     };
+    if (seed) {
+        knobs[0].second = around(base.bitmap_min, 64, 1u << 16);
+        knobs[1].second = around(base.chunk_elems, 4096, 1u << 22);
+        knobs[2].second = around(base.chunk_elems_bitmap, 1u << 16, 1u << 24);
+        knobs[3].second = {std::max(1u, base.probe_wg_per_cu / 2), base.probe_wg_per_cu, base.probe_wg_per_cu + 2,
+                           base.probe_wg_per_cu * 2};
+        knobs[4].second = around(base.ratio_search, 2, 1024);
+    }
     for (int pass = 0; pass < 2 && st == SI_OK; ++pass) {
         for (auto& kv : knobs) {
             for (uint32_t v : kv.second) {

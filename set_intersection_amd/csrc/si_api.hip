@@ -232,6 +232,11 @@ int si_batched_autotune(si_dataset* ds, double budget_ms, si_tuning* out) {
     return batched_autotune(ds, budget_ms, out);
 }
 
+int si_batched_autotune_seeded(si_dataset* ds, const si_tuning* seed, double budget_ms, si_tuning* out) {
+    if (!ds || !seed || !out) return fail(SI_ERR_ARG, "null argument");
+    return batched_autotune_seeded(ds, seed, budget_ms, out);
+}
+
 int si_edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, const si_tuning* tuning,
                         si_edge_plan** out) {
     if (!ds || !out || (m && (!u || !v))) return fail(SI_ERR_ARG, "null argument");

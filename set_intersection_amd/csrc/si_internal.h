@@ -183,6 +183,7 @@ int mm_tile_counts(si_dataset* ds, si_tile A, si_tile B, int32_t variant, uint64
 int batched_default_tuning(si_dataset* ds, si_tuning* t);
 int batched_count(si_dataset* ds, const si_tuning* tuning, uint32_t* d_out);
 int batched_autotune(si_dataset* ds, double budget_ms, si_tuning* best);
+int batched_autotune_seeded(si_dataset* ds, const si_tuning* seed, double budget_ms, si_tuning* best);
 int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, const si_tuning* tuning,
                      si_edge_plan** out);
 int edge_count(si_edge_plan* plan, uint32_t* d_counts, uint64_t* d_total);

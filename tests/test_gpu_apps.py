@@ -119,3 +119,41 @@ def test_pipelined_many_runs(exe, check_writers, tmp_path):
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
     assert filecmp.cmp(str(out), exp, shallow=False)
+
+
+def test_misi_batched_tune_file(check_writers, tmp_path):
+    """misi_batched --tune-file: a tuned run writes the dataset's record
+    (shape features + tuning), a second run uses the stored record, a
+    collection of similar shape (other seed) starts from the nearest record,
+    and a damaged store falls back to the defaults; every result file equals
+    the CPU writer's."""
+    from set_intersection_amd import datasets
+    store = str(tmp_path / "tune.json")
+    exe = os.path.join(BIN, "misi_batched")
+
+    def case(seed, *flags):
+        src = str(tmp_path / f"z{seed}.bin")
+        if not os.path.exists(src):
+            datasets.write_bin(datasets.zipf_collection(700, 1.2, 16, 4096, 1 << 16, seed), src)
+        exp = str(tmp_path / f"exp{seed}.txt")
+        if not os.path.exists(exp):
+            r = subprocess.run([check_writers, src, exp, "0"], capture_output=True, text=True, timeout=300)
+            assert r.returncode == 0, r.stderr
+        out = tmp_path / "got.txt"
+        r = subprocess.run([exe, "--input", src, "--output", str(out), "--partition", "100000", *flags],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+        assert filecmp.cmp(str(out), exp, shallow=False)
+        return r.stdout
+
+    assert "autotuned" in case(1, "--tune", "--tune-file", store)
+    text = open(store).read()
+    assert '"features"' in text and '"tuning"' in text and '"deg2"' in text
+    assert "stored" in case(1, "--tune-file", store) and "nearest" not in case(1, "--tune-file", store)
+    assert "nearest stored" in case(2, "--tune-file", store)
+    assert "autotuned from the stored record" in case(2, "--tune", "--tune-file", store)
+    assert open(store).read().count('"k=700') == 2  # one record per shape
+    with open(store, "w") as f:
+        f.write('{"k=700;ubits=17": {"features": {"k": 7')  # truncated
+    out = case(1, "--tune-file", store)
+    assert "default" in out

@@ -153,3 +153,38 @@ def test_degree_cdf(tmp_path):
     assert ec[-1, 1] == int((deg.astype(np.int64) ** 2).sum())
     sc = read("set_cdf")
     assert sc[-1, 1] == int(sum(deg[c.set(i)].sum() for i in range(c.k)))
+
+
+@pytest.fixture(scope="module")
+def check_tune_store(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("ts") / "check_tune_store")
+    r = run(["g++", "-O2", "-std=c++17", "-I" + HOST, "-I" + os.path.join(ROOT, "include"), "-o", exe,
+             os.path.join(ROOT, "tests", "cpp", "check_tune_store.cpp")])
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_tune_store(check_tune_store, tmp_path):
+    """The batched dispatcher's tuning store (host/tune_store.hpp): feature
+    closed form, save/load round trip, exact / nearest / no lookup, and a
+    damaged file whose bad records are skipped."""
+    r = run([check_tune_store, str(tmp_path)])
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("name", ["random_k12", "random_k23"])
+def test_tune_features_match_degree_cdf(check_tune_store, name, tmp_path):
+    """The element-degree features the store keys on are degree_cdf's
+    statistics: sum_e deg(e)^2 is the last value of its element CDF and the
+    distinct elements are the elements of degree > 0."""
+    src = os.path.join(GOLDEN, name + ".bin")
+    r = run([check_tune_store, str(tmp_path), src])
+    assert r.returncode == 0, r.stderr
+    distinct, deg_mean, deg2, total = r.stdout.split()
+    r = run([os.path.join(BIN, "degree_cdf"), src, str(tmp_path / "s")])
+    assert r.returncode == 0, r.stdout
+    ec = np.loadtxt(str(tmp_path / "s_element_cdf.txt"), dtype=np.int64, ndmin=2)
+    de = np.loadtxt(str(tmp_path / "s_degree_element.txt"), dtype=np.int64, ndmin=2)
+    assert int(distinct) == int((de[:, 0] > 0).sum())
+    assert abs(float(deg2) * int(total) - ec[-1, 1]) < 1e-6 * ec[-1, 1]
+    assert abs(float(deg_mean) * int(distinct) - int(total)) < 1e-9 * int(total)
