@@ -408,7 +408,7 @@ def bench_edges(args, si, datasets):
     rec["autotuned"] = {"ms": round(tms, 4), "tune_seconds": round(tune_s, 2),
                         "tuning": {f: int(getattr(tuning, f)) for f in ("bitmap_min", "chunk_elems",
                                                                         "chunk_elems_bitmap", "probe_wg_per_cu",
-                                                                        "ratio_search")}}
+                                                                        "ratio_search", "edge_bitmap_halves")}}
     ms = best_ms
     if not args.skip_plain:
         # the plain baseline on the same plan: one wave per edge, binary search

@@ -296,7 +296,12 @@ typedef struct si_tuning {
     uint32_t chunk_elems_bitmap; /* partner elements per work item (bitmap rows)  */
     uint32_t probe_wg_per_cu;    /* persistent workgroups per CU (hash rows)      */
     uint32_t use_graph;          /* replay the launches from a captured hipGraph  */
-    uint32_t reserved[9];
+    uint32_t edge_bitmap_halves; /* edge plans, universes above 2^19: 1 = bitmap
+                                    rows in universe halves (64 KiB bitmaps, two
+                                    workgroups per CU), 0 = one 128 KiB bitmap;
+                                    universes up to 2^19: 1 = 64 KiB bitmaps,
+                                    two workgroups per CU, 0 = one per CU      */
+    uint32_t reserved[8];
 } si_tuning;
 #define SI_TUNING_VERSION 2u
 int si_batched_default_tuning(si_dataset* ds, si_tuning* out);

@@ -43,7 +43,8 @@ class si_device_info(Structure):
 class si_tuning(Structure):
     _fields_ = [("version", c_uint32), ("bitmap_min", c_uint32), ("ratio_search", c_uint32),
                 ("chunk_elems", c_uint32), ("chunk_elems_bitmap", c_uint32), ("probe_wg_per_cu", c_uint32),
-                ("use_graph", c_uint32), ("reserved", c_uint32 * 9)]
+                ("use_graph", c_uint32), ("edge_bitmap_halves", c_uint32),
+                ("reserved", c_uint32 * 8)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}

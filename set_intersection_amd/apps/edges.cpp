@@ -88,7 +88,8 @@ int main(int argc, char** argv) {
                                   {"chunk_elems", si::toText(tuning.chunk_elems)},
                                   {"chunk_elems_bitmap", si::toText(tuning.chunk_elems_bitmap)},
                                   {"probe_wg_per_cu", si::toText(tuning.probe_wg_per_cu)},
-                                  {"ratio_search", si::toText(tuning.ratio_search)}});
+                                  {"ratio_search", si::toText(tuning.ratio_search)},
+                                  {"edge_bitmap_halves", si::toText(tuning.edge_bitmap_halves)}});
     }
     si::Interval* iv = ht.add("Edge plan");
     si_edge_plan* plan = nullptr;

@@ -50,6 +50,7 @@ struct Item {
 
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 constexpr int BM_NT = 1024, BM_WORDS = 32768;  // 128 KiB LDS bitmap: universes up to 2^20
+constexpr int HB_WORDS = 16384;                // 64 KiB LDS bitmap: a universe half, two workgroups per CU
 constexpr int PR_NT = 256, PR_WORDS = 8192;    // 32 KiB LDS: hash tables of <= 8192 slots
 constexpr uint32_t HASH_MAX = PR_WORDS / 2;    // largest row that fits a hash table
 constexpr int SEARCH_NS = 4096;                // samples cached for SEARCH rows
@@ -76,9 +77,10 @@ struct RowSmem {
     uint32_t pset[2 * 32], pcnt[2 * 32];
 };
 
+// BITMAP rows: bit x - base for each element x (all in [base, base + 32 * bmWords)).
 template <int NT, int LDSW>
 __device__ void build_row(RowSmem<LDSW>& sm, const uint32_t* __restrict__ S, uint64_t n, uint32_t mode,
-                          uint32_t bmWords) {
+                          uint32_t bmWords, uint32_t base = 0) {
     if (mode == MODE_BITMAP) {
         // 16-byte clears, then the sorted row's bits by wave chunks with eight
         // chunk loads in flight (plain stores inside a chunk, atomics at its
@@ -90,7 +92,7 @@ __device__ void build_row(RowSmem<LDSW>& sm, const uint32_t* __restrict__ S, uin
         constexpr uint32_t NW = NT / WAVE;
         const uint64_t chunks = (n + WAVE - 1) / WAVE;
         for (uint64_t c = threadIdx.x / WAVE; c < chunks; c += uint64_t(NW) * BUILD_UNROLL)
-            or_bits_chunks(S, n, c, NW, sm.tab);
+            or_bits_chunks(S, n, c, NW, sm.tab, base);
     } else if (mode == MODE_HASH) {
         uint32_t logTS = 6;
         while ((1ull << logTS) < 2 * n) ++logTS;
@@ -462,8 +464,10 @@ __device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uin
 // before ws, which is also the one when the whole share lies inside a single
 // partner; slot 1: the one that continues past we).  Partners of big rows
 // are long (hundreds of elements on average), so whole-wave runs waste few
-// lanes and need no position-to-partner search.
-template <int U, typename ProbeFn>
+// lanes and need no position-to-partner search.  ADD: whole partners' counts
+// are added to out (a partner counted in two universe halves) instead of
+// stored.
+template <int U, bool ADD, typename ProbeFn>
 __device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32_t j0, uint32_t j1, uint32_t ws,
                                           uint32_t we, const uint32_t* __restrict__ elems, ProbeFn probeFn,
                                           uint32_t* __restrict__ out, uint32_t* cutSet, uint32_t* cutCnt) {
@@ -504,7 +508,8 @@ __device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32
             }
             if (lane == 0) {
                 if (a == start && b == end) {
-                    out[uint32_t(__builtin_amdgcn_readlane(int(R.w), int(r)))] = cnt;
+                    uint32_t* o = out + uint32_t(__builtin_amdgcn_readlane(int(R.w), int(r)));
+                    if (ADD) atomicAdd(o, cnt); else *o = cnt;
                 } else {
                     const int slot = a != start ? 0 : 1;
                     cutSet[slot] = qb + r;
@@ -522,11 +527,18 @@ __device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32
 // persistent workgroups pop items; the waves split the item's partner
 // elements evenly and stream them partner by partner; partners cut by wave
 // boundaries are completed through LDS.
-template <int NT, int LDSW>
+// HALF: BITMAP items over one half of the universe (item mode bit 8: the
+// upper half [32 * LDSW, 64 * LDSW)).  The row's elements in the half are
+// [0, rowSplit[row]) or [rowSplit[row], n); the item's partner records
+// already hold only the partners' elements in the half.  A 64 KiB bitmap
+// lets two workgroups share a CU; each half adds its counts to out, which
+// the caller zeroes first.
+template <int NT, int LDSW, bool HALF = false>
 __global__ __launch_bounds__(NT) void k_edges_wg(const Item* __restrict__ items, uint32_t nitems,
                                                  uint32_t* __restrict__ queue, const uint32_t* __restrict__ elems,
                                                  const uint64_t* __restrict__ offsets, const uint4* __restrict__ part,
-                                                 uint32_t bmWords, uint32_t* __restrict__ out) {
+                                                 uint32_t bmWords, const uint32_t* __restrict__ rowSplit,
+                                                 uint32_t* __restrict__ out) {
     constexpr int NW = NT / WAVE;
     static_assert(NW <= 32, "partial slots");
     __shared__ RowSmem<LDSW> sm;
@@ -543,28 +555,43 @@ __global__ __launch_bounds__(NT) void k_edges_wg(const Item* __restrict__ items,
         const uint32_t it = sm.item;
         if (it >= nitems) break;
         const Item I = items[it];
-        const uint64_t rowOff = offsets[I.row], rowN = offsets[I.row + 1] - rowOff;
+        uint64_t rowOff = offsets[I.row], rowN = offsets[I.row + 1] - rowOff;
+        uint32_t base = 0;
+        if (HALF) {
+            const uint32_t sp = rowSplit[I.row];
+            if (I.mode >> 8) {
+                rowOff += sp;
+                rowN -= sp;
+                base = uint32_t(LDSW) * 32u;
+            } else {
+                rowN = sp;
+            }
+        }
         if (I.row != sm.row || I.mode != sm.mode) {
             __syncthreads();
-            build_row<NT, LDSW>(sm, elems + rowOff, rowN, I.mode, bmWords);
+            build_row<NT, LDSW>(sm, elems + rowOff, rowN, HALF ? uint32_t(MODE_BITMAP) : I.mode, bmWords, base);
             if (tid == 0) {
                 sm.row = I.row;
                 sm.mode = I.mode;
             }
         }
-        const uint32_t mode = I.mode, logTS = sm.logTS, hasMax = sm.hasMax;
+        const uint32_t mode = HALF ? uint32_t(MODE_BITMAP) : I.mode, logTS = sm.logTS, hasMax = sm.hasMax;
         const uint32_t* S = elems + rowOff;
         const uint32_t T = part[I.j1 - 1].z;
         const uint32_t ws = uint32_t((uint64_t(T) * w) / NW), we = uint32_t((uint64_t(T) * (w + 1)) / NW);
         if (ws < we) {
-            if (mode == MODE_BITMAP)  // the common case, without the generic probe's branches
-                edge_runs<4>(part, I.j0, I.j1, ws, we, elems,
-                             [&](uint32_t x) { return ((sm.tab[x >> 5] >> (x & 31)) & 1u) != 0u; }, out,
-                             &sm.pset[2 * w], &sm.pcnt[2 * w]);
-            else
-                edge_runs<4>(part, I.j0, I.j1, ws, we, elems,
-                             [&](uint32_t x) { return probe<LDSW>(sm, mode, x, S, rowN, logTS, hasMax) != 0u; },
-                             out, &sm.pset[2 * w], &sm.pcnt[2 * w]);
+            if (HALF || mode == MODE_BITMAP) {  // the common case, without the generic probe's branches
+                edge_runs<4, HALF>(part, I.j0, I.j1, ws, we, elems,
+                                   [&](uint32_t x) {
+                                       const uint32_t y = x - base;
+                                       return ((sm.tab[y >> 5] >> (y & 31)) & 1u) != 0u;
+                                   },
+                                   out, &sm.pset[2 * w], &sm.pcnt[2 * w]);
+            } else if (!HALF) {
+                edge_runs<4, false>(part, I.j0, I.j1, ws, we, elems,
+                                    [&](uint32_t x) { return probe<LDSW>(sm, mode, x, S, rowN, logTS, hasMax) != 0u; },
+                                    out, &sm.pset[2 * w], &sm.pcnt[2 * w]);
+            }
         }
         __syncthreads();
         if (tid == 0) {
@@ -577,7 +604,7 @@ __global__ __launch_bounds__(NT) void k_edges_wg(const Item* __restrict__ items,
                         tot += sm.pcnt[t];
                         if (t != s) sm.pset[t] = EMPTY;
                     }
-                out[part[j].w] = tot;
+                if (HALF) atomicAdd(&out[part[j].w], tot); else out[part[j].w] = tot;
             }
         }
         __syncthreads();
@@ -707,7 +734,28 @@ __global__ __launch_bounds__(NT) void k_edges_plain(const uint2* __restrict__ ed
 }
 
 __global__ void k_reset_queues(uint32_t* q) {
-    if (threadIdx.x < 4) q[threadIdx.x] = 0;
+    if (threadIdx.x < 5) q[threadIdx.x] = 0;
+}
+
+// lb[i] = the number of elements below key in list i (recs[i]: .x/.y element
+// offset, .z length; lists sorted).
+__global__ void k_lower_bounds(const uint4* __restrict__ recs, uint64_t n, const uint32_t* __restrict__ elems,
+                               uint32_t key, uint32_t* __restrict__ lb) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint4 r = recs[i];
+        const uint32_t* S = elems + ((uint64_t(r.y) << 32) | r.x);
+        uint32_t lo = 0, hi = r.z;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (S[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        lb[i] = lo;
+    }
+}
+
+__global__ void k_fill_zero(uint32_t* __restrict__ out, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        out[i] = 0;
 }
 
 }  // namespace
@@ -1078,16 +1126,21 @@ struct si_edge_plan {
     si_tuning t{};
     Item* d_wg = nullptr;    // BITMAP / HASH / SEARCH rows (workgroup items)
     Item* d_wave = nullptr;  // small HASH rows (wave items)
+    Item* d_half = nullptr;  // BITMAP rows by universe half (mode bit 8: upper half)
     uint4* d_part = nullptr;
+    uint4* d_hpart = nullptr;       // partner records of d_half items, elements in the item's half
+    uint32_t* d_rowsplit = nullptr;  // per row: its elements below the universe midpoint
     uint4* d_merge = nullptr;
     uint32_t* d_zero = nullptr;
     uint32_t* d_queue = nullptr;
     uint64_t* d_partials = nullptr;
     uint2* d_edges = nullptr;  // (u, v) per edge, for the plain baseline
-    uint32_t n_bm = 0, n_wg = 0, n_wave = 0, n_merge = 0, n_zero = 0;
+    uint32_t n_bm = 0, n_wg = 0, n_wave = 0, n_merge = 0, n_zero = 0, n_half = 0;
+    bool bm64 = false;  // whole-universe bitmaps fit 64 KiB: two bitmap workgroups per CU
     ~si_edge_plan() {
-        for (void* p : {(void*)d_wg, (void*)d_wave, (void*)d_part, (void*)d_merge, (void*)d_zero, (void*)d_queue,
-                        (void*)d_partials, (void*)d_edges})
+        for (void* p : {(void*)d_wg, (void*)d_wave, (void*)d_half, (void*)d_part, (void*)d_hpart,
+                        (void*)d_rowsplit, (void*)d_merge, (void*)d_zero, (void*)d_queue, (void*)d_partials,
+                        (void*)d_edges})
             if (p) (void)hipFree(p);
     }
 };
@@ -1133,22 +1186,27 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
         }
     }
     const bool bmFits = bitmap_words(ds) <= uint32_t(BM_WORDS);
-    std::vector<Item> bmItems, wgItems, waveItems;
-    std::vector<uint4> part, merge;
+    // universe halves: bitmap rows split at 32 * HB_WORDS, each half an item of its own
+    const bool halves = bmFits && t.edge_bitmap_halves && bitmap_words(ds) > uint32_t(HB_WORDS);
+    plan->bm64 = t.edge_bitmap_halves && bitmap_words(ds) <= uint32_t(HB_WORDS);
+    std::vector<Item> bmItems, wgItems, waveItems, halfItems;
+    std::vector<uint4> part, merge, hraw, hpart;
     std::vector<uint32_t> zero;
+    std::vector<uint3> hrows;  // (row, first, end) of its records in hraw
     part.reserve(m);
-    // partners [p0, p1) of `part` (already appended) cut into items of about
-    // `chunk` partner elements; part[q].z becomes the inclusive end within its item
-    auto emit = [&](std::vector<Item>& vec, uint32_t row, uint32_t p0, uint32_t p1, uint32_t mode, uint64_t chunk) {
+    // partners [p0, p1) of `pv` (already appended) cut into items of about
+    // `chunk` partner elements; pv[q].z becomes the inclusive end within its item
+    auto emit = [&](std::vector<uint4>& pv, std::vector<Item>& vec, uint32_t row, uint32_t p0, uint32_t p1,
+                    uint32_t mode, uint64_t chunk) {
         uint32_t p = p0;
         while (p < p1) {
             uint64_t acc = 0;
             uint32_t e = p;
-            while (e < p1 && (e == p || acc + part[e].z <= chunk)) acc += part[e++].z;
+            while (e < p1 && (e == p || acc + pv[e].z <= chunk)) acc += pv[e++].z;
             uint32_t run = 0;
             for (uint32_t q = p; q < e; ++q) {
-                run += part[q].z;
-                part[q].z = run;
+                run += pv[q].z;
+                pv[q].z = run;
             }
             vec.push_back(Item{row, p, e, mode});
             p = e;
@@ -1172,16 +1230,29 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
             for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) zero.push_back(plist[q].y);
             continue;
         }
-        if (useBm || s <= HASH_MAX) {
+        if (useBm && halves) {
+            const uint32_t h0 = uint32_t(hraw.size());
+            for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) {
+                const uint2 pe = plist[q];
+                const uint32_t n = sz[pe.x];
+                if (n == 0) {
+                    zero.push_back(pe.y);
+                    continue;
+                }
+                const uint64_t o = of[pe.x];
+                hraw.push_back(make_uint4(uint32_t(o), uint32_t(o >> 32), n, pe.y));
+            }
+            if (uint32_t(hraw.size()) > h0) hrows.push_back(make_uint3(r, h0, uint32_t(hraw.size())));
+        } else if (useBm || s <= HASH_MAX) {
             for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) add(plist[q]);
             const uint32_t end = uint32_t(part.size());
             if (end == start) continue;
             if (useBm)
-                emit(bmItems, r, start, end, MODE_BITMAP, t.chunk_elems_bitmap);
+                emit(part, bmItems, r, start, end, MODE_BITMAP, t.chunk_elems_bitmap);
             else if (s <= WAVE_HASH_MAX)
-                emit(waveItems, r, start, end, MODE_HASH, t.chunk_elems);
+                emit(part, waveItems, r, start, end, MODE_HASH, t.chunk_elems);
             else
-                emit(wgItems, r, start, end, MODE_HASH, t.chunk_elems);
+                emit(part, wgItems, r, start, end, MODE_HASH, t.chunk_elems);
         } else {
             for (uint32_t q = cnt[r]; q < cnt[r + 1]; ++q) {
                 const uint2 pe = plist[q];
@@ -1193,10 +1264,62 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
                     add(pe);
             }
             const uint32_t end = uint32_t(part.size());
-            if (end > start) emit(wgItems, r, start, end, MODE_SEARCH, t.chunk_elems);
+            if (end > start) emit(part, wgItems, r, start, end, MODE_SEARCH, t.chunk_elems);
         }
     }
     si_context* ctx = ds->ctx;
+    std::vector<uint32_t> rowSplit;
+    if (!hrows.empty()) {
+        // each list's elements below the midpoint, by a device binary search
+        // (partner lists, then the rows)
+        std::vector<uint4> recs(hraw);
+        for (const uint3& h : hrows) {
+            const uint64_t o = of[h.x];
+            recs.push_back(make_uint4(uint32_t(o), uint32_t(o >> 32), sz[h.x], 0u));
+        }
+        const uint64_t nrec = recs.size();
+        uint4* d_recs = nullptr;
+        uint32_t* d_lb = nullptr;
+        std::vector<uint32_t> lb(nrec);
+        int st = SI_OK;
+        if (hipMalloc(reinterpret_cast<void**>(&d_recs), nrec * sizeof(uint4)) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&d_lb), nrec * 4) != hipSuccess)
+            st = fail(SI_ERR_NOMEM, "edge plan: device memory for the universe-half split");
+        if (st == SI_OK &&
+            hipMemcpyAsync(d_recs, recs.data(), nrec * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            st = fail(SI_ERR_HIP, "edge plan: upload of the split records");
+        if (st == SI_OK) {
+            const uint32_t grid = uint32_t(std::min<uint64_t>((nrec + 255) / 256, 4096));
+            hipLaunchKernelGGL(k_lower_bounds, dim3(grid), dim3(256), 0, ctx->stream, d_recs, nrec, ds->d_elems,
+                               uint32_t(HB_WORDS) * 32u, d_lb);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(lb.data(), d_lb, nrec * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                st = fail(SI_ERR_HIP, "edge plan: universe-half split");
+        }
+        if (d_recs) (void)hipFree(d_recs);
+        if (d_lb) (void)hipFree(d_lb);
+        SI_TRY(st);
+        rowSplit.assign(k, 0u);
+        hpart.reserve(2 * hraw.size());
+        for (size_t i = 0; i < hrows.size(); ++i) {
+            const uint3 h = hrows[i];
+            rowSplit[h.x] = lb[hraw.size() + i];
+            for (uint32_t half = 0; half < 2; ++half) {
+                const uint32_t start = uint32_t(hpart.size());
+                for (uint32_t q = h.y; q < h.z; ++q) {
+                    const uint4 R = hraw[q];
+                    const uint32_t sp = lb[q], nh = half ? R.z - sp : sp;
+                    if (nh == 0) continue;  // no elements in this half: adds 0 to a zeroed count
+                    const uint64_t o = ((uint64_t(R.y) << 32) | R.x) + (half ? sp : 0u);
+                    hpart.push_back(make_uint4(uint32_t(o), uint32_t(o >> 32), nh, R.w));
+                }
+                const uint32_t end = uint32_t(hpart.size());
+                if (end > start)
+                    emit(hpart, halfItems, h.x, start, end, MODE_BITMAP | (half << 8), t.chunk_elems_bitmap);
+            }
+        }
+    }
     auto up = [&](auto& vec, auto*& dptr) -> int {
         using T = typename std::decay<decltype(vec)>::type::value_type;
         SI_HIP(hipMalloc(reinterpret_cast<void**>(&dptr), std::max<size_t>(vec.size(), 1) * sizeof(T)));
@@ -1210,6 +1333,9 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
     SI_TRY(up(bmItems, plan->d_wg));
     SI_TRY(up(waveItems, plan->d_wave));
     SI_TRY(up(part, plan->d_part));
+    SI_TRY(up(halfItems, plan->d_half));
+    SI_TRY(up(hpart, plan->d_hpart));
+    SI_TRY(up(rowSplit, plan->d_rowsplit));
     SI_TRY(up(merge, plan->d_merge));
     SI_TRY(up(zero, plan->d_zero));
     SI_HIP(hipMalloc(&plan->d_queue, 256));
@@ -1225,6 +1351,7 @@ int edge_plan_create(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint6
     plan->n_wave = uint32_t(waveItems.size());
     plan->n_merge = uint32_t(merge.size());
     plan->n_zero = uint32_t(zero.size());
+    plan->n_half = uint32_t(halfItems.size());
     ds->ctx->edge_plans.insert(plan.get());
     *out = plan.release();
     return SI_OK;
@@ -1234,7 +1361,12 @@ int edge_count(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
     si_dataset* ds = P->ds;
     si_context* ctx = ds->ctx;
     hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, ctx->stream, P->d_queue);
-    if (P->n_zero) {
+    if (P->n_half) {
+        // universe halves add their counts: every count starts at zero (this
+        // also covers the empty-list edges)
+        const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((P->m + 255) / 256, 4096)));
+        hipLaunchKernelGGL(k_fill_zero, dim3(grid), dim3(256), 0, ctx->stream, counts, P->m);
+    } else if (P->n_zero) {
         const uint32_t grid = uint32_t(std::min<uint64_t>((P->n_zero + 255) / 256, 1024));
         hipLaunchKernelGGL(k_zero_list, dim3(grid), dim3(256), 0, ctx->stream, P->d_zero, P->n_zero, counts);
     }
@@ -1254,17 +1386,30 @@ int edge_count(si_edge_plan* P, uint32_t* counts, uint64_t* total) {
         hipLaunchKernelGGL(k_edges_wave, dim3(grid), dim3(WH_NT), 0, s, P->d_wave, P->n_wave, P->d_queue + 3,
                            ds->d_elems, ds->d_offsets, P->d_part, counts);
     }
-    if (P->n_bm) {
+    if (P->n_half) {
+        KernelScope ks(ctx, "edges_rows_bitmap_half");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * 2, P->n_half));
+        hipLaunchKernelGGL((k_edges_wg<BM_NT, HB_WORDS, true>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_half,
+                           P->n_half, P->d_queue + 4, ds->d_elems, ds->d_offsets, P->d_hpart, uint32_t(HB_WORDS),
+                           P->d_rowsplit, counts);
+    }
+    if (P->n_bm && P->bm64) {
+        KernelScope ks(ctx, "edges_rows_bitmap");
+        const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * 2, P->n_bm));
+        hipLaunchKernelGGL((k_edges_wg<BM_NT, HB_WORDS>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_wg, P->n_bm,
+                           P->d_queue + 0, ds->d_elems, ds->d_offsets, P->d_part, bitmap_words(ds), nullptr, counts);
+    } else if (P->n_bm) {
         KernelScope ks(ctx, "edges_rows_bitmap");
         const uint32_t grid = uint32_t(std::min<uint64_t>(ctx->num_cus, P->n_bm));
         hipLaunchKernelGGL((k_edges_wg<BM_NT, BM_WORDS>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_wg, P->n_bm,
-                           P->d_queue + 0, ds->d_elems, ds->d_offsets, P->d_part, bitmap_words(ds), counts);
+                           P->d_queue + 0, ds->d_elems, ds->d_offsets, P->d_part, bitmap_words(ds), nullptr, counts);
     }
     if (P->n_wg) {
         KernelScope ks(ctx, "edges_rows_probe");
         const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * P->t.probe_wg_per_cu, P->n_wg));
         hipLaunchKernelGGL((k_edges_wg<PR_NT, PR_WORDS>), dim3(grid), dim3(PR_NT), 0, ctx->stream, P->d_wg + P->n_bm,
-                           P->n_wg, P->d_queue + 1, ds->d_elems, ds->d_offsets, P->d_part, bitmap_words(ds), counts);
+                           P->n_wg, P->d_queue + 1, ds->d_elems, ds->d_offsets, P->d_part, bitmap_words(ds), nullptr,
+                           counts);
     }
     if (forked) {
         SI_HIP(hipEventRecord(ctx->join, ctx->side));
@@ -1339,6 +1484,7 @@ int edge_autotune(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t
         {&si_tuning::chunk_elems, {32768, 131072}},
         {&si_tuning::probe_wg_per_cu, {2, 8}},
         {&si_tuning::ratio_search, {16, 64}},
+        {&si_tuning::edge_bitmap_halves, {0, 1}},
     };
     for (auto& kv : knobs) {
         for (uint32_t val : kv.second) {

@@ -285,8 +285,9 @@ __device__ __forceinline__ void or_bits_chunk(uint32_t x, bool valid, uint32_t* 
 // loads in flight.
 constexpr int BUILD_UNROLL = 8;
 
+// Bits are set at x - base (a bitmap over [base, base + 32 * words)).
 __device__ __forceinline__ void or_bits_chunks(const uint32_t* __restrict__ S, uint64_t n, uint64_t cb,
-                                               uint64_t stride, uint32_t* __restrict__ bm) {
+                                               uint64_t stride, uint32_t* __restrict__ bm, uint32_t base = 0) {
     const uint32_t lane = lane_id();
     uint32_t x[BUILD_UNROLL];
     bool v[BUILD_UNROLL];
@@ -294,7 +295,7 @@ __device__ __forceinline__ void or_bits_chunks(const uint32_t* __restrict__ S, u
     for (int u = 0; u < BUILD_UNROLL; ++u) {
         const uint64_t i = (cb + uint64_t(u) * stride) * WAVE + lane;
         v[u] = i < n;
-        x[u] = v[u] ? S[i] : 0u;
+        x[u] = v[u] ? S[i] - base : 0u;
     }
 #pragma unroll
     for (int u = 0; u < BUILD_UNROLL; ++u)

@@ -155,6 +155,59 @@ def test_edges_bins(ctx, bitmap_min, ratio):
     assert np.array_equal(got, exp) and total == etot
 
 
+def _halves_collection():
+    """Lists over [0, 2^20) that straddle the universe midpoint 2^19 in every
+    way: all below, all above, elements at 2^19 - 1, 2^19 and 2^20 - 1, long
+    rows cut into several items, and an empty list."""
+    rng = np.random.default_rng(11)
+    H = 1 << 19
+    sets = []
+    for i, n in enumerate([5000, 3000, 2000, 1200, 900, 700, 600, 520, 300, 64, 40, 7, 1, 0]):
+        if i % 4 == 1:
+            x = rng.integers(0, H, n)                # lower half only
+        elif i % 4 == 2:
+            x = rng.integers(H, 2 * H, n)            # upper half only
+        else:
+            x = rng.integers(0, 2 * H, n)
+        sets.append(np.unique(x).astype(np.uint32))
+    sets[0] = np.unique(np.concatenate([sets[0], [H - 1, H, 2 * H - 1, 0]])).astype(np.uint32)
+    sets[5] = np.unique(np.concatenate([sets[5], [H - 1]])).astype(np.uint32)  # ends just below the midpoint
+    sets[6] = np.unique(np.concatenate([sets[6], [H]])).astype(np.uint32)      # starts at the midpoint
+    sets.append(sets[0][::3].copy())                                            # a subset of a row
+    return si.Collection.from_sets(sets)
+
+
+@pytest.mark.parametrize("shift", [0, 1])
+def test_edges_bitmap_halves(ctx, shift):
+    """Bitmap rows in 64 KiB bitmaps, two workgroups per CU
+    (edge_bitmap_halves = 1): universe halves with the partners' lists split
+    at 2^19 (shift 0, universe 2^20), and whole-universe 64 KiB bitmaps
+    (shift 1: the same lists halved into a 2^19 universe).  Counts equal the
+    oracle's and the 128 KiB path's, with small work items (rows cut into
+    several items, partners cut by wave boundaries), twice on one plan."""
+    coll = _halves_collection()
+    if shift:
+        coll = si.Collection.from_sets([np.unique(s >> 1).astype(np.uint32) for s in (coll.set(i) for i in range(coll.k))])
+    k = coll.k
+    rng = np.random.default_rng(5)
+    u = rng.integers(0, k, 4000).astype(np.uint32)
+    v = rng.integers(0, k, 4000).astype(np.uint32)
+    ds = ctx.upload(coll)
+    exp, etot = oracle.edges(coll, u, v)
+    for chunk in (1 << 20, 700):
+        t = tuning(ds, bitmap_min=512, chunk_elems_bitmap=chunk, edge_bitmap_halves=1)
+        plan = ds.edge_plan(u, v, t)
+        counts = ctx.buffer(4 * u.size)
+        total = ctx.buffer(8)
+        for _ in range(2):
+            plan.count_async(counts, total)
+            assert np.array_equal(counts.download(np.uint32), exp), chunk
+            assert int(total.download(np.uint64)[0]) == etot
+        got0, tot0 = ds.edge_count(u, v, tuning(ds, bitmap_min=512, chunk_elems_bitmap=chunk,
+                                                edge_bitmap_halves=0))
+        assert np.array_equal(got0, exp) and tot0 == etot
+
+
 def test_edges_arbitrary_pairs(ctx):
     coll = datasets.zipf_collection(200, 1.1, 1, 70000, 2**32, 6)  # big rows, no bitmap
     rng = np.random.default_rng(0)
