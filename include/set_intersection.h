@@ -227,6 +227,10 @@ int si_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* out,
                         uint64_t cap, uint64_t* n);
 /* Device form: writes the list to `out` (device) and the size as uint64 at
  * byte offset 0 of `count_out`.  `out` must hold min(|S_a|,|S_b|) elements.
+ * Sets of similar size are merged (the merge-path kernel of si_pair_count
+ * with SI_TECH_IP); when one set is at least 64 times the other, the smaller
+ * set's elements are searched in the larger one instead (the obs search of
+ * obs.cu:296-345, with a decoupled look-back placing the matches in order).
  * A dataset uploaded with validate = 0 is validated (once, synchronously) by
  * its first materialisation: SI_ERR_UNSORTED if a set is not strictly
  * ascending. */

@@ -45,14 +45,6 @@ namespace {
 using namespace si_dev;
 
 // ---------------------------------------------------------------- SISI ----
-// state[s] of a materialising launch: published flag (bit 63) | inclusive
-// flag (bit 62) | count (bits 0-32; a pair's intersection is below 2^33).
-// 0 means "not published"; the last workgroup clears the array, so launches
-// need no memset (and replay correctly from a captured graph).
-constexpr uint64_t kCountMask = (1ull << 33) - 1;
-constexpr uint64_t kPublished = 1ull << 63;
-constexpr uint64_t kInclusive = 1ull << 62;
-
 // One workgroup per merge-path segment of the pair (A, B).
 // Count mode: partials[s] holds segment s's count; the last workgroup to
 // finish (ticket done[0]) folds them into out / out32 and resets done[0].
@@ -100,37 +92,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
     if (MAT) {
         if (threadIdx.x < WAVE) {
-            // wave 0 looks back 64 predecessors per round: lane l reads
-            // state[j - l]; the closest inclusive state ends the walk once
-            // every state up to it is published (before segment 0 there is
-            // an implicit inclusive 0)
-            const uint32_t lane = lane_id();
-            // the counts are the whole payload: relaxed atomics suffice, and
-            // polling without acquire keeps the CU's L1 intact for the
-            // workgroups still merging beside this one
-            if (lane == 0) __hip_atomic_store(&state[s], kPublished | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint64_t pre = 0;
-            int64_t j = int64_t(s) - 1;
-            while (j >= 0) {
-                const int64_t idx = j - int64_t(lane);
-                const uint64_t v = idx >= 0 ? __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                            : (kPublished | kInclusive);
-                const uint64_t incl = __ballot((v & kInclusive) != 0);
-                const uint64_t pub = __ballot((v & kPublished) != 0);
-                // lanes [0, last] must be published: up to the closest inclusive, else all 64
-                const uint32_t last = incl ? uint32_t(__builtin_ctzll(incl)) : WAVE - 1;
-                const uint64_t need = last == WAVE - 1 ? ~0ull : ((2ull << last) - 1);
-                if ((pub & need) != need) {
-                    __builtin_amdgcn_s_sleep(16);
-                    continue;
-                }
-                pre += wave_sum_u64(lane <= last ? (v & kCountMask) : 0ull);
-                if (incl) break;
-                j -= WAVE;
-            }
-            if (lane == 0) {
-                __hip_atomic_store(&state[s], kPublished | kInclusive | (pre + total), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t pre = lookback_exclusive(state, s, total);
+            if (lane_id() == 0) {
                 shPrefix = pre;
                 if (s == nseg - 1 && out) *out = pre + total;
             }

@@ -215,6 +215,112 @@ __device__ __forceinline__ uint32_t sample_search(const uint32_t* __restrict__ S
     return 0;
 }
 
+// ------------------------------------------------------ decoupled look-back
+// state[s] of segment s of a materialising launch: published flag (bit 63) |
+// inclusive flag (bit 62) | count (bits 0-32; a pair's intersection is below
+// 2^33).  0 means "not published"; the launch's last workgroup clears the
+// array, so launches need no memset (and replay correctly from a captured
+// graph).
+constexpr uint64_t kLbCountMask = (1ull << 33) - 1;
+constexpr uint64_t kLbPublished = 1ull << 63;
+constexpr uint64_t kLbInclusive = 1ull << 62;
+
+// Called by all 64 lanes of one wave of segment s's workgroup: publishes the
+// segment's count, adds up its predecessors' counts 64 states per round
+// (lane l reads state[j - l]; the closest inclusive state ends the walk once
+// every state up to it is published; before segment 0 there is an implicit
+// inclusive 0), publishes the segment's inclusive prefix and returns the
+// exclusive one.  Segments must be numbered in the order they start (a
+// ticket), so that every predecessor is running or done.  The counts are the
+// whole payload: relaxed atomics suffice, and polling without acquire keeps
+// the CU's L1 intact for the workgroups still working beside this one.
+__device__ __forceinline__ uint64_t lookback_exclusive(uint64_t* __restrict__ state, uint32_t s, uint64_t total) {
+    const uint32_t lane = lane_id();
+    if (lane == 0) __hip_atomic_store(&state[s], kLbPublished | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t pre = 0;
+    int64_t j = int64_t(s) - 1;
+    while (j >= 0) {
+        const int64_t idx = j - int64_t(lane);
+        const uint64_t v = idx >= 0 ? __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : (kLbPublished | kLbInclusive);
+        const uint64_t incl = __ballot((v & kLbInclusive) != 0);
+        const uint64_t pub = __ballot((v & kLbPublished) != 0);
+        // lanes [0, last] must be published: up to the closest inclusive, else all 64
+        const uint32_t last = incl ? uint32_t(__builtin_ctzll(incl)) : WAVE - 1;
+        const uint64_t need = last == WAVE - 1 ? ~0ull : ((2ull << last) - 1);
+        if ((pub & need) != need) {
+            __builtin_amdgcn_s_sleep(16);
+            continue;
+        }
+        pre += wave_sum_u64(lane <= last ? (v & kLbCountMask) : 0ull);
+        if (incl) break;
+        j -= WAVE;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&state[s], kLbPublished | kLbInclusive | (pre + total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    return pre;
+}
+
+// sample_search for R keys at once (lanes with valid[r] false report no
+// match): the phases run in lockstep over the keys, so each step issues R
+// independent LDS or global loads.  Bit r of the result: x[r] is in S.
+template <int NS, int R>
+__device__ __forceinline__ uint32_t sample_search_multi(const uint32_t* __restrict__ S, uint64_t nS,
+                                                        const uint32_t* cache, const uint32_t (&x)[R],
+                                                        const bool (&valid)[R]) {
+    static_assert((NS & (NS - 1)) == 0, "NS must be a power of two");
+    uint32_t lo[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) lo[r] = 0;
+    // phase 1: lo = number of samples <= x (branch-free over a power-of-two NS)
+    constexpr int LOG2NS = __builtin_ctz(NS);
+#pragma unroll
+    for (int k = LOG2NS - 1; k >= 0; --k) {
+        const uint32_t step = 1u << k;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (cache[lo[r] + step - 1] <= x[r]) lo[r] += step;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (lo[r] < NS && cache[lo[r]] <= x[r]) lo[r] += 1;
+    uint32_t hit = 0, active = 0;
+    uint64_t l[R], h[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        l[r] = h[r] = 0;
+        if (!valid[r] || lo[r] == 0) continue;  // x < S[0]
+        if (cache[lo[r] - 1] == x[r]) {
+            hit |= 1u << r;
+            continue;
+        }
+        // phase 2: S[(j-1)*nS/NS] < x < S[j*nS/NS]  (or the end of S)
+        l[r] = (uint64_t(lo[r] - 1) * nS) / NS + 1;
+        h[r] = (lo[r] == NS) ? nS : (uint64_t(lo[r]) * nS) / NS;
+        if (l[r] < h[r]) active |= 1u << r;
+    }
+    while (active) {
+        uint32_t y[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if ((active >> r) & 1u) y[r] = S[(l[r] + h[r]) >> 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!((active >> r) & 1u)) continue;
+            const uint64_t mid = (l[r] + h[r]) >> 1;
+            if (y[r] == x[r]) {
+                hit |= 1u << r;
+                active &= ~(1u << r);
+                continue;
+            }
+            if (y[r] < x[r]) l[r] = mid + 1; else h[r] = mid;
+            if (l[r] >= h[r]) active &= ~(1u << r);
+        }
+    }
+    return hit;
+}
+
 // ------------------------------------------------------------ pair indexing
 // triangular_index of include/util.hpp:68-70 in 64-bit arithmetic.
 __device__ __forceinline__ uint64_t tri_index(uint64_t n, uint64_t i, uint64_t j) {

@@ -160,6 +160,8 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
 int ip_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
 // optimized binary search (k_obs.hip)
 int obs_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);
+// Materialisation by search of the smaller set's elements in the larger one.
+int obs_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out, uint64_t* d_count);
 int obs_run(si_dataset* ds, const si_params* p, si_tile A, si_tile B, uint32_t n, uint32_t* d_counts);
 // hash intersect (k_hash.hip)
 int hi_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, uint64_t* d_out, uint32_t* d_out32);

@@ -173,6 +173,48 @@ def test_materialise_unvalidated(ctx):
     assert np.array_equal(ds2.pair_materialize(0, 1), oracle.materialize(good.set(0), good.set(1)))
 
 
+def _skewed_pairs():
+    """(large, small) pairs 32x to 1024x apart: the materialisation searches
+    the small set's elements in the large one.  Small-set elements below the
+    large set's first, above its last, equal to both ends, a subset (every
+    probe matches), a disjoint set (none does), and probe sets spanning
+    several chunks per workgroup."""
+    rng = np.random.default_rng(17)
+    big = np.unique(rng.integers(1000, 2**32 - 1000, 1 << 22, dtype=np.uint64)).astype(np.uint32)
+    yield big, np.unique(np.concatenate([rng.choice(big, 3000), rng.integers(0, 2**32, 3000, dtype=np.uint64),
+                                         [0, 999, big[0], big[-1], 2**32 - 1]]).astype(np.uint32))
+    yield big, big[::33].copy()                                                  # every probe matches
+    yield big, np.setdiff1d(big[::40].astype(np.uint64) + 1, big).astype(np.uint32)  # no probe matches
+    yield big, np.sort(rng.choice(big, 99_997, replace=False))                     # a ragged last chunk
+    huge = np.unique(rng.integers(0, 2**32, 1 << 25, dtype=np.uint64)).astype(np.uint32)
+    yield huge, np.unique(np.concatenate([rng.choice(huge, 1 << 19), rng.integers(0, 2**32, 1 << 19,
+                                                                                  dtype=np.uint64)])).astype(np.uint32)
+
+
+def test_materialise_skewed(ctx, monkeypatch):
+    """Pairs whose sizes are at least 32x apart materialise by search
+    (k_obs_mat): the list equals the oracle's in both argument orders, twice
+    in a row (the look-back states and tickets reset themselves), and equals
+    the merge path's (forced with SI_SISI_SHAPE)."""
+    monkeypatch.setenv("SI_SISI_SHAPE", "1")
+    merge_ctx = si.Context(0)
+    try:
+        for big, small in _skewed_pairs():
+            assert big.size // small.size >= 32
+            coll = si.Collection.from_sets([big, small])
+            exp = oracle.materialize(big, small)
+            ds = ctx.upload(coll)
+            for a, b in ((0, 1), (1, 0), (0, 1)):
+                got = ds.pair_materialize(a, b)
+                assert got.size == exp.size and np.array_equal(got, exp), (big.size, small.size, a)
+            ds.free()
+            mds = merge_ctx.upload(coll)
+            assert np.array_equal(mds.pair_materialize(1, 0), exp)
+            mds.free()
+    finally:
+        merge_ctx.close()
+
+
 # ------------------------------------------------------------------ MISI --
 @pytest.mark.parametrize("name", ["closed_form", "random_k12", "random_k23"])
 @pytest.mark.parametrize("partition", [3, 5, 10000])
