@@ -208,10 +208,11 @@ int si_pair_count_async(si_dataset* ds, uint32_t a, uint32_t b, int32_t techniqu
  * and result.  No counterpart in the reference; it is for callers that
  * cannot allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc).  The count kernels' scratch (one uint64
- * per workgroup plus a completion ticket) comes from a file-scope device
- * array of SI_NOALLOC_SCRATCH_BYTES per GPU, set up by si_open, instead of
+ * per workgroup plus a completion ticket; for obs also the searched set's
+ * 16 KiB of gathered samples) comes from a file-scope device array of
+ * SI_NOALLOC_SCRATCH_BYTES per GPU, set up by si_open, instead of
  * library-allocated memory; no allocator is called.  A launch that would need
- * more scratch (more than 8160 workgroups, e.g. from a large
+ * more scratch (more than 8160 workgroups, 6112 for obs, e.g. from a large
  * si_params.blocks) fails with SI_ERR_RANGE.  All calls on one GPU share the
  * array: calls that overlap in time on one GPU (other contexts, other host
  * threads) race and must be serialised by the caller. */

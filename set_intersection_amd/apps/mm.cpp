@@ -4,11 +4,12 @@
 // result file (writeResult<T, true>, io.hpp:104-136), written run by run as
 // the runs end (host/pipeline.hpp).  The incidence matrices
 // are built on the device as bit rows and multiplied on the matrix cores:
-// --variant auto (default) takes the fp4 MFMA path (FP32 accumulation, exact
-// while counts stay below 2^24, as the reference's SGEMM) when the universe is
-// below 2^24 and the int8 path (int32 accumulation) otherwise; i8, fp4, f32
-// (fp8 MFMA, FP32 accumulation) and popcount (bitmap AND + popcount) select a
-// path explicitly.
+// --variant auto (default) and i8 take the int8 MFMA path with int32
+// accumulation (exact for every count below 2^31); fp4 (e2m1 MFMA) and f32
+// (fp8 MFMA) accumulate in FP32 and are exact while counts stay below 2^24,
+// as the reference's SGEMM (they refuse larger universes); popcount is the
+// bitmap AND + popcount kernel.  fp4 is the fastest of them and is kept as a
+// checked comparison beside the int8 path.
 #include <algorithm>
 #include <cstdio>
 #include <string>
@@ -59,7 +60,7 @@ int main(int argc, char** argv) {
         return 1;
     }
     int32_t v = variant == "f32" ? SI_MM_F32 : variant == "popcount" ? SI_MM_POPCOUNT : variant == "i8" ? SI_MM_I8
-              : variant == "fp4" ? SI_MM_FP4 : variant == "auto" ? SI_MM_FP4 : -1;
+              : variant == "fp4" ? SI_MM_FP4 : variant == "auto" ? SI_MM_I8 : -1;
     if (v < 0 || partition == 0) {
         std::printf("bad --variant or --partition\n");
         si_close(ctx);
@@ -82,7 +83,6 @@ int main(int argc, char** argv) {
     std::vector<si::tile_pair> runs = si::findTilePairs(d.cardinality, partition);
     const uint64_t combinations = uint64_t(partition) * partition;
     const uint64_t K = uint64_t(d.universe) + 1;
-    if (variant == "auto" && K > (uint64_t(1) << 24)) v = SI_MM_I8;  // FP32 counts would round
     const uint64_t deviceMemory = (K * partition * 2) + combinations * 4;
     si::printTable("Launch info", {{"Partition", si::toText(partition)},
                                    {"GPU invocations", si::toText(runs.size())},
