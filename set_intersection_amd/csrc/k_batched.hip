@@ -356,7 +356,32 @@ __global__ void k_gather(const uint32_t* __restrict__ elems, const uint64_t* __r
 // partner's elements within its item, output index}.  Items are (row,
 // partner range [j0, j1), mode); the item's partner elements are numbered
 // 0 .. end[j1-1] in partner order (a virtual concatenation).
-//
+
+// First partner record q in [j0, j1) whose elements end past position ws
+// (part[q].z > ws; part[j1 - 1].z > ws holds), found by the whole wave: each
+// round tests 64 records spread over the candidates, so n partners take
+// ceil(log64 n) dependent rounds instead of log2 n.  Partners have at least
+// one element, so ws = 0 needs no search.
+__device__ __forceinline__ uint32_t first_partner(const uint4* __restrict__ part, uint32_t j0, uint32_t j1,
+                                                  uint32_t ws) {
+    if (ws == 0) return j0;
+    const uint32_t lane = lane_id();
+    uint32_t lo = j0, hi = j1 - 1;  // the answer lies in [lo, hi]; hi satisfies the test
+    while (lo < hi) {
+        const uint32_t n = hi - lo, step = (n + WAVE - 1) / WAVE;
+        const uint32_t idx = lo + lane * step;
+        const uint64_t b = __ballot(idx < hi && part[idx].z > ws);
+        if (b) {  // the test is monotone: the first passing lane bounds the answer from above
+            const uint32_t f = uint32_t(__builtin_ctzll(b));
+            hi = lo + f * step;
+            lo = f ? lo + (f - 1) * step + 1 : lo;
+        } else {  // every tested record fails: the answer follows the last one
+            lo += ((n - 1) / step) * step + 1;
+        }
+    }
+    return lo;
+}
+
 // Lanes hold 64 consecutive partners of a wave's element range; every
 // element position finds its partner by a shuffle binary search over the
 // lanes' start positions, is probed against the row structure, and each lane
@@ -364,18 +389,12 @@ __global__ void k_gather(const uint32_t* __restrict__ elems, const uint64_t* __r
 // partner wholly inside [ws, we) gets its count stored; one cut by ws or we
 // leaves its part in cutSet/cutCnt (slot 0: began before ws, slot 1: goes on
 // past we).  Partners must have at least one element.
-
 template <typename ProbeFn>
 __device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uint32_t j0, uint32_t j1, uint32_t ws,
                                              uint32_t we, const uint32_t* __restrict__ elems, ProbeFn probeFn,
                                              uint32_t* __restrict__ out, uint32_t* cutSet, uint32_t* cutCnt) {
     const uint32_t lane = lane_id();
-    // first partner whose elements end past ws
-    uint32_t lo = j0, hi = j1 - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (part[mid].z > ws) hi = mid; else lo = mid + 1;
-    }
+    const uint32_t lo = first_partner(part, j0, j1, ws);
     uint32_t qc = lo, carry = 0;
     uint32_t c0 = ws;
     while (c0 < we) {
@@ -472,12 +491,7 @@ __device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32
                                           uint32_t we, const uint32_t* __restrict__ elems, ProbeFn probeFn,
                                           uint32_t* __restrict__ out, uint32_t* cutSet, uint32_t* cutCnt) {
     const uint32_t lane = lane_id();
-    // first partner whose elements end past ws
-    uint32_t lo = j0, hi = j1 - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (part[mid].z > ws) hi = mid; else lo = mid + 1;
-    }
+    const uint32_t lo = first_partner(part, j0, j1, ws);
     uint32_t qb = lo;
     uint32_t prevEnd = (qb == j0) ? 0u : part[qb - 1].z;
     while (qb < j1 && prevEnd < we) {
