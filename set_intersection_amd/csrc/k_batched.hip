@@ -70,6 +70,29 @@ __device__ __forceinline__ uint64_t bit_range(int lo, int hi) {
     return h & ~l;
 }
 
+// First index j in [lo, hi) with pass(j), or hi if none (pass monotone:
+// false ... false true ... true), found by the whole wave: each round tests
+// 64 indices spread over the candidates, so n candidates take ceil(log64 n)
+// dependent rounds instead of the log2 n of a binary search.  All lanes call
+// it with the same arguments.
+template <typename Pass>
+__device__ __forceinline__ uint32_t wave_first(uint32_t lo, uint32_t hi, Pass pass) {
+    const uint32_t lane = lane_id();
+    while (lo < hi) {  // the answer lies in [lo, hi]
+        const uint32_t n = hi - lo, step = (n + WAVE - 1) / WAVE;
+        const uint32_t idx = lo + lane * step;
+        const uint64_t b = __ballot(idx < hi && pass(idx));
+        if (b) {  // the first passing lane bounds the answer from above
+            const uint32_t f = uint32_t(__builtin_ctzll(b));
+            hi = lo + f * step;
+            lo = f ? lo + (f - 1) * step + 1 : lo;
+        } else {  // every tested index fails: the answer follows the last one
+            lo += ((n - 1) / step) * step + 1;
+        }
+    }
+    return lo;
+}
+
 template <int LDSW>
 struct RowSmem {
     alignas(16) uint32_t tab[LDSW];
@@ -196,13 +219,8 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
         const uint64_t e0 = soff[I.j0], e1 = soff[I.j1], len = e1 - e0;
         const uint64_t ws = e0 + (len * w) / NW, we = e0 + (len * (w + 1)) / NW;
         if (ws < we) {
-            // first set: largest j in [j0, j1) with soff[j] <= ws
-            uint32_t lo = I.j0, hi = I.j1;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (soff[mid] <= ws) lo = mid; else hi = mid;
-            }
-            uint32_t jc = lo;
+            // first set: largest j in [j0, j1) with soff[j] <= ws (soff[j0] = e0 <= ws)
+            uint32_t jc = wave_first(I.j0 + 1, I.j1, [&](uint32_t j) { return soff[j] > ws; }) - 1;
             uint32_t carry = 0;
             for (uint64_t c0 = ws; c0 < we; c0 += WAVE * VT) {
                 const uint64_t c1 = (c0 + WAVE * VT < we) ? c0 + WAVE * VT : we;
@@ -358,28 +376,12 @@ __global__ void k_gather(const uint32_t* __restrict__ elems, const uint64_t* __r
 // 0 .. end[j1-1] in partner order (a virtual concatenation).
 
 // First partner record q in [j0, j1) whose elements end past position ws
-// (part[q].z > ws; part[j1 - 1].z > ws holds), found by the whole wave: each
-// round tests 64 records spread over the candidates, so n partners take
-// ceil(log64 n) dependent rounds instead of log2 n.  Partners have at least
-// one element, so ws = 0 needs no search.
+// (part[q].z > ws; part[j1 - 1].z > ws holds).  Partners have at least one
+// element, so ws = 0 needs no search.
 __device__ __forceinline__ uint32_t first_partner(const uint4* __restrict__ part, uint32_t j0, uint32_t j1,
                                                   uint32_t ws) {
     if (ws == 0) return j0;
-    const uint32_t lane = lane_id();
-    uint32_t lo = j0, hi = j1 - 1;  // the answer lies in [lo, hi]; hi satisfies the test
-    while (lo < hi) {
-        const uint32_t n = hi - lo, step = (n + WAVE - 1) / WAVE;
-        const uint32_t idx = lo + lane * step;
-        const uint64_t b = __ballot(idx < hi && part[idx].z > ws);
-        if (b) {  // the test is monotone: the first passing lane bounds the answer from above
-            const uint32_t f = uint32_t(__builtin_ctzll(b));
-            hi = lo + f * step;
-            lo = f ? lo + (f - 1) * step + 1 : lo;
-        } else {  // every tested record fails: the answer follows the last one
-            lo += ((n - 1) / step) * step + 1;
-        }
-    }
-    return lo;
+    return wave_first(j0, j1, [&](uint32_t q) { return part[q].z > ws; });
 }
 
 // Lanes hold 64 consecutive partners of a wave's element range; every
