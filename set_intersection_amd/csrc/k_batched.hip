@@ -189,6 +189,11 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
     constexpr int NW = NT / WAVE;
     static_assert(NW <= 32, "partial slots");
     __shared__ RowSmem<LDSW> sm;
+    // per wave and chunk: the hit mask of word v and the hits before it
+    // (entry VT: no bits, the chunk's total), so a set's count over chunk
+    // positions [a, b) is hitsBefore(b) - hitsBefore(a)
+    __shared__ uint64_t wmask[NW][VT + 1];
+    __shared__ uint32_t wpre[NW][VT + 1];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
     if (tid == 0) {
         sm.row = EMPTY;
@@ -259,6 +264,30 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                         M[v] = __ballot(hit);
                     }
                 }
+                // the chunk's masks and running hit counts, lane v holding word v
+                {
+                    uint32_t mlo = 0, mhi = 0, pre = 0, run = 0;
+#pragma unroll
+                    for (int v = 0; v < VT; ++v) {
+                        const bool mine = lane == uint32_t(v);
+                        mlo = mine ? uint32_t(M[v]) : mlo;
+                        mhi = mine ? uint32_t(M[v] >> 32) : mhi;
+                        pre = mine ? run : pre;
+                        run += uint32_t(__popcll(M[v]));
+                    }
+                    if (lane == uint32_t(VT)) pre = run;  // mlo = mhi = 0 there
+                    if (lane <= uint32_t(VT)) {
+                        wmask[w][lane] = (uint64_t(mhi) << 32) | mlo;
+                        wpre[w][lane] = pre;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                auto hitsBefore = [&](uint32_t x) {  // x in [0, 64 VT]
+                    const uint32_t v = x >> 6, r = x & 63u;
+                    return wpre[w][v] + uint32_t(__popcll(wmask[w][v] & ((1ull << r) - 1ull)));
+                };
                 // sets overlapping [c0, c1): jc, jc+1, ...; lane t takes jc + g*64 + t
                 uint32_t nextJc = jc, nextCarry = 0;
                 for (uint32_t g = 0;; g += WAVE) {
@@ -272,15 +301,9 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                     if (valid) {
                         const uint64_t sBeg = soff[j];
                         sEnd = soff[j + 1];
-                        const int64_t a = int64_t((sBeg > c0 ? sBeg : c0) - c0);
-                        const int64_t b = int64_t((sEnd < c1 ? sEnd : c1) - c0);
-#pragma unroll
-                        for (int v = 0; v < VT; ++v) {
-                            int lo2 = int(a - 64 * v), hi2 = int(b - 64 * v);
-                            lo2 = lo2 < 0 ? 0 : (lo2 > 64 ? 64 : lo2);
-                            hi2 = hi2 < 0 ? 0 : (hi2 > 64 ? 64 : hi2);
-                            if (hi2 > lo2) cnt += __popcll(M[v] & bit_range(lo2, hi2));
-                        }
+                        const uint32_t a = uint32_t((sBeg > c0 ? sBeg : c0) - c0);
+                        const uint32_t b = uint32_t((sEnd < c1 ? sEnd : c1) - c0);
+                        cnt = hitsBefore(b) - hitsBefore(a);
                         if (g == 0 && lane == 0) cnt += carry;
                         endsHere = sEnd <= c1;
                         if (endsHere) {
