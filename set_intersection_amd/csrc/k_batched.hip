@@ -63,13 +63,6 @@ __device__ __forceinline__ uint64_t pair_slot(uint32_t k, uint32_t pa, uint32_t 
     return tri_index(k, lo, hi);
 }
 
-// bits [lo, hi) of a 64-bit word (0 <= lo <= hi <= 64)
-__device__ __forceinline__ uint64_t bit_range(int lo, int hi) {
-    const uint64_t h = hi >= 64 ? ~0ull : ((1ull << hi) - 1);
-    const uint64_t l = lo >= 64 ? ~0ull : ((1ull << lo) - 1);
-    return h & ~l;
-}
-
 // First index j in [lo, hi) with pass(j), or hi if none (pass monotone:
 // false ... false true ... true), found by the whole wave: each round tests
 // 64 indices spread over the candidates, so n candidates take ceil(log64 n)
@@ -91,6 +84,38 @@ __device__ __forceinline__ uint32_t wave_first(uint32_t lo, uint32_t hi, Pass pa
         }
     }
     return lo;
+}
+
+// A wave's chunk of 64 x VT positions is probed into VT ballot masks (bit l
+// of word v: position 64 v + l).  The wave stores the masks and the hits
+// before each word (entry VT: no bits, the chunk's total) in its own LDS
+// arrays (VT + 1 entries each), so that the hits at chunk positions [a, b)
+// are chunk_hits_before(b) - chunk_hits_before(a): two LDS reads and two
+// popcounts per lane instead of masking all VT words.
+__device__ __forceinline__ void store_chunk_masks(const uint64_t (&M)[VT], uint64_t* wm, uint32_t* wp) {
+    const uint32_t lane = lane_id();
+    uint32_t mlo = 0, mhi = 0, pre = 0, run = 0;  // lane v takes word v
+#pragma unroll
+    for (int v = 0; v < VT; ++v) {
+        const bool mine = lane == uint32_t(v);
+        mlo = mine ? uint32_t(M[v]) : mlo;
+        mhi = mine ? uint32_t(M[v] >> 32) : mhi;
+        pre = mine ? run : pre;
+        run += uint32_t(__popcll(M[v]));
+    }
+    if (lane == uint32_t(VT)) pre = run;
+    if (lane <= uint32_t(VT)) {
+        wm[lane] = (uint64_t(mhi) << 32) | mlo;
+        wp[lane] = pre;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t chunk_hits_before(const uint64_t* wm, const uint32_t* wp, uint32_t x) {
+    const uint32_t v = x >> 6, r = x & 63u;  // x in [0, 64 VT]
+    return wp[v] + uint32_t(__popcll(wm[v] & ((1ull << r) - 1ull)));
 }
 
 template <int LDSW>
@@ -189,10 +214,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
     constexpr int NW = NT / WAVE;
     static_assert(NW <= 32, "partial slots");
     __shared__ RowSmem<LDSW> sm;
-    // per wave and chunk: the hit mask of word v and the hits before it
-    // (entry VT: no bits, the chunk's total), so a set's count over chunk
-    // positions [a, b) is hitsBefore(b) - hitsBefore(a)
-    __shared__ uint64_t wmask[NW][VT + 1];
+    __shared__ uint64_t wmask[NW][VT + 1];  // store_chunk_masks, per wave
     __shared__ uint32_t wpre[NW][VT + 1];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
     if (tid == 0) {
@@ -264,30 +286,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                         M[v] = __ballot(hit);
                     }
                 }
-                // the chunk's masks and running hit counts, lane v holding word v
-                {
-                    uint32_t mlo = 0, mhi = 0, pre = 0, run = 0;
-#pragma unroll
-                    for (int v = 0; v < VT; ++v) {
-                        const bool mine = lane == uint32_t(v);
-                        mlo = mine ? uint32_t(M[v]) : mlo;
-                        mhi = mine ? uint32_t(M[v] >> 32) : mhi;
-                        pre = mine ? run : pre;
-                        run += uint32_t(__popcll(M[v]));
-                    }
-                    if (lane == uint32_t(VT)) pre = run;  // mlo = mhi = 0 there
-                    if (lane <= uint32_t(VT)) {
-                        wmask[w][lane] = (uint64_t(mhi) << 32) | mlo;
-                        wpre[w][lane] = pre;
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                }
-                auto hitsBefore = [&](uint32_t x) {  // x in [0, 64 VT]
-                    const uint32_t v = x >> 6, r = x & 63u;
-                    return wpre[w][v] + uint32_t(__popcll(wmask[w][v] & ((1ull << r) - 1ull)));
-                };
+                store_chunk_masks(M, wmask[w], wpre[w]);
                 // sets overlapping [c0, c1): jc, jc+1, ...; lane t takes jc + g*64 + t
                 uint32_t nextJc = jc, nextCarry = 0;
                 for (uint32_t g = 0;; g += WAVE) {
@@ -303,7 +302,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                         sEnd = soff[j + 1];
                         const uint32_t a = uint32_t((sBeg > c0 ? sBeg : c0) - c0);
                         const uint32_t b = uint32_t((sEnd < c1 ? sEnd : c1) - c0);
-                        cnt = hitsBefore(b) - hitsBefore(a);
+                        cnt = chunk_hits_before(wmask[w], wpre[w], b) - chunk_hits_before(wmask[w], wpre[w], a);
                         if (g == 0 && lane == 0) cnt += carry;
                         endsHere = sEnd <= c1;
                         if (endsHere) {
@@ -417,7 +416,8 @@ __device__ __forceinline__ uint32_t first_partner(const uint4* __restrict__ part
 template <typename ProbeFn>
 __device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uint32_t j0, uint32_t j1, uint32_t ws,
                                              uint32_t we, const uint32_t* __restrict__ elems, ProbeFn probeFn,
-                                             uint32_t* __restrict__ out, uint32_t* cutSet, uint32_t* cutCnt) {
+                                             uint32_t* __restrict__ out, uint32_t* cutSet, uint32_t* cutCnt,
+                                             uint64_t* wm, uint32_t* wp) {
     const uint32_t lane = lane_id();
     const uint32_t lo = first_partner(part, j0, j1, ws);
     uint32_t qc = lo, carry = 0;
@@ -468,18 +468,13 @@ __device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uin
             M[v] = __ballot(hit);
         }
         // this lane's partner covers [start, end); its positions in [c0, c1)
+        store_chunk_masks(M, wm, wp);
         uint32_t cnt = 0;
         const bool touches = valid && start < c1 && end > c0;
         if (touches) {
-            const int64_t a = int64_t(start > c0 ? start : c0) - int64_t(c0);
-            const int64_t b = int64_t(end < c1 ? end : c1) - int64_t(c0);
-#pragma unroll
-            for (int v = 0; v < VT; ++v) {
-                int l2 = int(a - 64 * v), h2 = int(b - 64 * v);
-                l2 = l2 < 0 ? 0 : (l2 > 64 ? 64 : l2);
-                h2 = h2 < 0 ? 0 : (h2 > 64 ? 64 : h2);
-                if (h2 > l2) cnt += __popcll(M[v] & bit_range(l2, h2));
-            }
+            const uint32_t a = (start > c0 ? start : c0) - c0;
+            const uint32_t b = (end < c1 ? end : c1) - c0;
+            cnt = chunk_hits_before(wm, wp, b) - chunk_hits_before(wm, wp, a);
             if (lane == 0) cnt += carry;
             if (end <= c1) {
                 if (start >= ws && end <= we) out[P.w] = cnt;                  // whole partner in this wave
@@ -661,6 +656,8 @@ __global__ __launch_bounds__(WH_NT) void k_edges_wave(const Item* __restrict__ i
                                                       const uint64_t* __restrict__ offsets,
                                                       const uint4* __restrict__ part, uint32_t* __restrict__ out) {
     __shared__ uint32_t tabs[WH_NT / WAVE][WH_SLOTS];
+    __shared__ uint64_t wmask[WH_NT / WAVE][VT + 1];  // store_chunk_masks, per wave
+    __shared__ uint32_t wpre[WH_NT / WAVE][VT + 1];
     const uint32_t lane = lane_id();
     uint32_t* tab = tabs[threadIdx.x / WAVE];
     while (true) {
@@ -707,7 +704,7 @@ __global__ __launch_bounds__(WH_NT) void k_edges_wave(const Item* __restrict__ i
                                  sl = (sl + 1) & mask;
                              }
                          },
-                         out, nullptr, nullptr);
+                         out, nullptr, nullptr, wmask[threadIdx.x / WAVE], wpre[threadIdx.x / WAVE]);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
