@@ -252,31 +252,32 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
             for (uint64_t c0 = ws; c0 < we; c0 += WAVE * VT) {
                 const uint64_t c1 = (c0 + WAVE * VT < we) ? c0 + WAVE * VT : we;
                 uint64_t M[VT];
+                // chunk position q = 64 v + lane from a uniform chunk base and a
+                // 32-bit offset; positions past c1 re-read c1 - 1 (masked off)
+                const uint32_t* cb = selem + c0;
+                const uint32_t last = uint32_t(c1 - c0) - 1;
                 if (mode == MODE_BITMAP) {
-                    // the common case, branch-free: positions past c1 re-read c1 - 1
+                    // the common case, branch-free
                     uint32_t x[VT];
 #pragma unroll
                     for (int v = 0; v < VT; ++v) {
-                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
-                        x[v] = selem[p < c1 ? p : c1 - 1];
+                        const uint32_t q = uint32_t(v) * WAVE + lane;
+                        x[v] = cb[q < last ? q : last];
                     }
 #pragma unroll
-                    for (int v = 0; v < VT; ++v) {
-                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
-                        M[v] = __ballot(((sm.tab[x[v] >> 5] >> (x[v] & 31)) & 1u) && p < c1);
-                    }
+                    for (int v = 0; v < VT; ++v)
+                        M[v] = __ballot(((sm.tab[x[v] >> 5] >> (x[v] & 31)) & 1u) && uint32_t(v) * WAVE + lane <= last);
                 } else if (mode == MODE_HASH) {
                     uint32_t x[VT];
 #pragma unroll
                     for (int v = 0; v < VT; ++v) {
-                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
-                        x[v] = selem[p < c1 ? p : c1 - 1];
+                        const uint32_t q = uint32_t(v) * WAVE + lane;
+                        x[v] = cb[q < last ? q : last];
                     }
 #pragma unroll
-                    for (int v = 0; v < VT; ++v) {
-                        const uint64_t p = c0 + uint64_t(v) * WAVE + lane;
-                        M[v] = __ballot(probe<LDSW>(sm, MODE_HASH, x[v], S, rowN, logTS, hasMax) && p < c1);
-                    }
+                    for (int v = 0; v < VT; ++v)
+                        M[v] = __ballot(uint32_t(v) * WAVE + lane <= last &&
+                                        probe<LDSW>(sm, MODE_HASH, x[v], S, rowN, logTS, hasMax));
                 } else {
 #pragma unroll
                     for (int v = 0; v < VT; ++v) {
