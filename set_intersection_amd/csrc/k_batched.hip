@@ -143,6 +143,8 @@ __device__ void build_row(RowSmem<LDSW>& sm, const uint32_t* __restrict__ S, uin
             or_bits_chunks(S, n, c, NW, sm.tab, base);
     } else if (mode == MODE_HASH) {
         uint32_t logTS = 6;
+        // load <= 1/4 where the LDS region allows (short probe chains), else <= 1/2
+        while ((1ull << logTS) < 4 * n && (1ull << logTS) < uint64_t(LDSW)) ++logTS;
         while ((1ull << logTS) < 2 * n) ++logTS;
         const uint32_t TS = 1u << logTS;
         for (uint32_t i = threadIdx.x; i < TS; i += NT) sm.tab[i] = EMPTY;
@@ -670,6 +672,7 @@ __global__ __launch_bounds__(WH_NT) void k_edges_wave(const Item* __restrict__ i
         const uint64_t ro = offsets[I.row];
         const uint32_t rn = uint32_t(offsets[I.row + 1] - ro);
         uint32_t logTS = 6;
+        while ((1u << logTS) < 4 * rn && (1u << logTS) < uint32_t(WH_SLOTS)) ++logTS;  // load <= 1/4 if it fits
         while ((1u << logTS) < 2 * rn) ++logTS;
         const uint32_t TS = 1u << logTS, mask = TS - 1;
         for (uint32_t i = lane; i < TS; i += WAVE) tab[i] = EMPTY;
