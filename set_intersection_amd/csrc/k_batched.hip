@@ -143,8 +143,8 @@ __device__ void build_row(RowSmem<LDSW>& sm, const uint32_t* __restrict__ S, uin
             or_bits_chunks(S, n, c, NW, sm.tab, base);
     } else if (mode == MODE_HASH) {
         uint32_t logTS = 6;
-        // load <= 1/8 where the LDS region allows (short probe chains), else <= 1/2
-        while ((1ull << logTS) < 8 * n && (1ull << logTS) < uint64_t(LDSW)) ++logTS;
+        // load <= 1/16 where the LDS region allows (short probe chains), else <= 1/2
+        while ((1ull << logTS) < 16 * n && (1ull << logTS) < uint64_t(LDSW)) ++logTS;
         while ((1ull << logTS) < 2 * n) ++logTS;
         const uint32_t TS = 1u << logTS;
         for (uint32_t i = threadIdx.x; i < TS; i += NT) sm.tab[i] = EMPTY;
