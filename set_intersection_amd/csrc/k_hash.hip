@@ -137,8 +137,12 @@ __global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict_
             const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
             for (uint64_t p0 = s0; p0 < s1; p0 += PIECE) {
                 const uint32_t m = uint32_t(s1 - p0 < PIECE ? s1 - p0 : PIECE);
-                // 2^lg groups of 4 slots, at least 2m slots: load <= 1/2
-                uint32_t lg = 32u - uint32_t(__builtin_clz(2 * m - 1));
+                // 2^lg groups of 4 slots, at least 8m slots where TSMAX allows
+                // (load <= 1/8: most probes end in their first group), else
+                // 2m (m <= TSMAX / 2: load <= 1/2)
+                constexpr uint32_t LGMAX = uint32_t(__builtin_ctz(uint32_t(TSMAX)));
+                uint32_t lg = 32u - uint32_t(__builtin_clz(8 * m - 1));
+                lg = lg < LGMAX ? lg : LGMAX;
                 lg = (lg < 6 ? 6 : lg) - 2;
                 const uint32_t gmask = (1u << lg) - 1u;
                 for (uint32_t i = threadIdx.x; i <= gmask; i += NT) tab4[i] = make_uint4(0u, 0u, 0u, 0u);
