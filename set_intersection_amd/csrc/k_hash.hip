@@ -86,7 +86,7 @@ __global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ Q, uint64_t nQ, uint32_t bits,
                                                      uint64_t nb, uint32_t per, uint64_t* __restrict__ partials,
                                                      uint32_t* __restrict__ done, uint64_t* __restrict__ out64,
-                                                     uint32_t* __restrict__ out32) {
+                                                     uint32_t* __restrict__ out32, uint32_t prio) {
     static_assert(TSMAX / 2 <= NT * U, "the first piece of a bucket's S elements comes from registers");
     extern __shared__ uint64_t bnd[];  // S bounds [0, nbd), Q bounds [nbd, 2 nbd)
     __shared__ uint4 tab4[TSMAX / 4];
@@ -131,7 +131,9 @@ __global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict_
         uint32_t xs[U], xq[U], ys[U], yq[U];
         uint32_t k = work(0);
         if (k + 1 < nbd) fetch(k, xs, xq);
+        uint32_t step = 0;
         while (k + 1 < nbd) {
+            if (prio) set_wave_prio((step++ + blockIdx.x) & 3u);
             const uint32_t kn = work(k + 1);
             if (kn + 1 < nbd) fetch(kn, ys, yq);
             const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
@@ -232,6 +234,7 @@ __global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict_
             k = kn;
         }
     }
+    if (prio) set_wave_prio(0);
     grid_fold_u64<NT>(c, partials, done, out64, out32, red);
 }
 
@@ -307,7 +310,7 @@ int hi_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, ui
     hipLaunchKernelGGL((k_hash_buckets<PAIR_NT, 16, PAIR_TSMAX>), dim3(uint32_t(g)), dim3(PAIR_NT), lds, ctx->stream,
                        ds->d_elems + ds->offsets[s], nS, ds->d_elems + ds->offsets[q], nQ, uint32_t(bits), nb,
                        uint32_t(per), reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256),
-                       static_cast<uint32_t*>(ws), d_out, d_out32);
+                       static_cast<uint32_t*>(ws), d_out, d_out32, uint32_t(ctx->sisi_prio));
     SI_HIP(hipGetLastError());
     return SI_OK;
 }

@@ -61,7 +61,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                     uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
                                                     uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
                                                     uint32_t* __restrict__ stage, uint64_t stage_cap,
-                                                    uint32_t* __restrict__ list, uint64_t* __restrict__ state) {
+                                                    uint32_t* __restrict__ list, uint64_t* __restrict__ state,
+                                                    uint32_t prio) {
     __shared__ StreamSmem<NT, VT, WN, MAT ? 1024 : 1> sm;
     __shared__ uint32_t shTicket;
     __shared__ uint64_t shPrefix;
@@ -87,7 +88,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
     uint32_t* slot = MAT ? stage + uint64_t(s) * stage_cap : nullptr;
     const uint64_t c =
-        merge_segment_stream<NT, VT, WN, MAT, SEL, MAT ? 1024 : 1>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm, slot);
+        merge_segment_stream<NT, VT, WN, MAT, SEL, MAT ? 1024 : 1>(base, oA, nA, oB, nB, ia, ib, iaE, ibE, sm, slot,
+                                                             prio ? s : kNoPrio);
     const uint64_t total = block_sum_u64<NT>(c, sm.red);
 
     if (MAT) {
@@ -232,7 +234,7 @@ template <int NT, int VT, int WN, bool SEL, bool MAT>
 static void launch_stream(const SisiLaunch& L) {
     hipLaunchKernelGGL((k_sisi_stream<NT, VT, WN, MAT, SEL>), dim3(L.nseg), dim3(NT), 0, L.ctx->stream, L.base, L.oA,
                        L.nA, L.oB, L.nB, L.seg_len, L.nseg, L.partials, L.done, L.out, L.out32, L.stage, L.cap,
-                       L.list, L.state);
+                       L.list, L.state, uint32_t(L.ctx->sisi_prio));
 }
 
 template <bool MAT>

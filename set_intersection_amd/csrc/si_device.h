@@ -12,6 +12,22 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Wave issue priority (s_setprio takes an immediate).  Kernels that stream
+// one long range per workgroup, several workgroups per CU, rotate it per
+// tile or bucket (priority (step + seed) mod 4): a CU otherwise issues from
+// its oldest waves first, so the first of its workgroups runs ahead and the
+// launch ends with the last ones (SISI merge at 2^28 x 2^28: materialise
+// 0.553 -> 0.534 ms, count 0.441 -> 0.420 ms).
+constexpr uint32_t kNoPrio = 0xffffffffu;
+__device__ __forceinline__ void set_wave_prio(uint32_t p) {
+    switch (p) {
+        case 0: __builtin_amdgcn_s_setprio(0); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        default: __builtin_amdgcn_s_setprio(3); break;
+    }
+}
+
 // --------------------------------------------------------------- reductions
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll

@@ -83,6 +83,7 @@ struct si_context {
     si_impl::DevMem ws_aux2;
     bool ws_small_zeroed = false;
     int sisi_shape = 0;  // tile shape of the SISI merge kernel used by materialisation
+    int sisi_prio = 1;   // SISI merge kernel rotates wave priority by tile (SI_SISI_PRIO=0 turns it off)
     // si_pair_count_noalloc: this device's copy of the file-scope scratch
     // array, and whether the current call must take its workspace from it
     void* noalloc_ws = nullptr;

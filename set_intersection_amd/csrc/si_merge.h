@@ -423,7 +423,8 @@ __device__ __forceinline__ uint32_t split_search(const uint32_t* wa, uint32_t sa
 template <int NT, int VT, int WN, bool MAT, bool SEL, int HB>
 __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA, uint64_t oB,
                                          uint64_t nB, uint64_t ia, uint64_t ib, uint64_t iaEnd, uint64_t ibEnd,
-                                         StreamSmem<NT, VT, WN, HB>& sm, uint32_t* __restrict__ stage) {
+                                         StreamSmem<NT, VT, WN, HB>& sm, uint32_t* __restrict__ stage,
+                                         uint32_t prioSeed = kNoPrio) {
     using C = StreamCfg<NT, VT, WN>;
     constexpr uint32_t T = C::T, W = C::W, M = C::M;
     const uint32_t tid = threadIdx.x;
@@ -461,7 +462,9 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
     // search the whole range
     constexpr uint32_t kBackoff = 8;
     uint32_t shareQ = 1u << 15, fullLeft = 0;
+    uint32_t tile = 0;
     while (d < dEnd) {
+        if (prioSeed != kNoPrio) set_wave_prio((tile + prioSeed) & 3u);
         const uint64_t rem64 = dEnd - d;
         const uint32_t rem = uint32_t(rem64 < T ? rem64 : T);
         // a side is cut when the window cannot hold all that remains of it;
@@ -688,11 +691,13 @@ __device__ uint64_t merge_segment_stream(const uint32_t* __restrict__ base, uint
         xA4 = nxA4;
         xB4 = nxB4;
         if (Tc) shareQ = uint32_t((uint64_t(ca) << 16) / Tc);
+        ++tile;
         ia += ca;
         ib += cb;
         d += Tc;
         __syncthreads();
     }
+    if (prioSeed != kNoPrio) set_wave_prio(0);
     return myCount;
 }
 
