@@ -47,7 +47,8 @@ template <int NT, int NS, int R>
 __global__ __launch_bounds__(NT) void k_obs_pair(const uint32_t* __restrict__ base, uint64_t oS, uint64_t nS,
                                                  uint64_t oP, uint64_t nP, const uint32_t* __restrict__ samples,
                                                  uint64_t* __restrict__ partials, uint32_t* __restrict__ done,
-                                                 uint64_t* __restrict__ out64, uint32_t* __restrict__ out32) {
+                                                 uint64_t* __restrict__ out64, uint32_t* __restrict__ out32,
+                                                 uint32_t prio) {
     __shared__ alignas(16) uint32_t cache[NS];
     __shared__ uint64_t red[NT / WAVE];
     const uint32_t* S = base + oS;
@@ -57,7 +58,9 @@ __global__ __launch_bounds__(NT) void k_obs_pair(const uint32_t* __restrict__ ba
     __syncthreads();
     uint64_t c = 0;
     const uint64_t stride = uint64_t(gridDim.x) * NT * R;
+    uint32_t step = 0;
     for (uint64_t i0 = uint64_t(blockIdx.x) * NT * R + threadIdx.x; i0 < nP; i0 += stride) {
+        if (prio) set_wave_prio((step++ + blockIdx.x) & 3u);
         uint32_t x[R];
         bool v[R];
 #pragma unroll
@@ -68,6 +71,7 @@ __global__ __launch_bounds__(NT) void k_obs_pair(const uint32_t* __restrict__ ba
         }
         c += uint32_t(__builtin_popcount(sample_search_multi<NS, R>(S, nS, cache, x, v)));
     }
+    if (prio) set_wave_prio(0);
     grid_fold_u64<NT>(c, partials, done, out64, out32, red);
 }
 
@@ -217,7 +221,7 @@ int obs_pair_count(si_dataset* ds, uint32_t a, uint32_t b, const si_params* p, u
                        nS, samples);
     hipLaunchKernelGGL((k_obs_pair<PAIR_NT, PAIR_NS, PAIR_R>), dim3(uint32_t(grid)), dim3(PAIR_NT), 0, ctx->stream,
                        ds->d_elems, oS, nS, oP, nP, samples, reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + 256),
-                       static_cast<uint32_t*>(ws), d_out, d_out32);
+                       static_cast<uint32_t*>(ws), d_out, d_out32, uint32_t(ctx->sisi_prio));
     SI_HIP(hipGetLastError());
     return SI_OK;
 }
