@@ -303,6 +303,29 @@ def test_tile_shapes(shape, monkeypatch):
         c.close()
 
 
+@pytest.mark.parametrize("prio", [0, 1])
+def test_priority_rotation(prio, monkeypatch):
+    """The SISI merge, obs, hi and bitmap_dynamic kernels rotate their waves'
+    issue priority per tile, probe round or bucket (SI_SISI_PRIO, read at
+    context open; on by default): counts and lists are exact either way."""
+    monkeypatch.setenv("SI_SISI_PRIO", str(prio))
+    c = si.Context(0)
+    try:
+        pairs = list(_adversarial_pairs())
+        pairs.append((datasets.uniform_sorted(3_000_000, 1 << 26, 5), datasets.uniform_sorted(2_500_000, 1 << 26, 6)))
+        for a, b in pairs:
+            coll = si.Collection.from_sets([a, b])
+            ds = c.upload(coll)
+            exp = oracle.materialize(coll.set(0), coll.set(1))
+            for t in (si.TECH_IP, si.TECH_OBS, si.TECH_HI, si.TECH_BITMAP_DYNAMIC):
+                assert ds.pair_count(0, 1, technique=t) == exp.size, (prio, t, a.size, b.size)
+                assert ds.pair_count(1, 0, technique=t) == exp.size, (prio, t, a.size, b.size)
+            assert np.array_equal(ds.pair_materialize(0, 1), exp), (prio, a.size, b.size)
+            ds.free()
+    finally:
+        c.close()
+
+
 def test_sisi_full_size():
     """The headline pair at full size: |A| = |B| = 2^28 from [0, 2^32).
     Exact against the oracle's std::set_intersection; the materialised list is
