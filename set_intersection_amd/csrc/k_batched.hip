@@ -86,6 +86,30 @@ __device__ __forceinline__ uint32_t wave_first(uint32_t lo, uint32_t hi, Pass pa
     return lo;
 }
 
+// v_writelane_b32: lane L of `old` replaced by the wave-uniform value `val`
+// (no compare, no select)
+template <int L>
+__device__ __forceinline__ uint32_t write_lane(uint32_t old, uint32_t val) {
+    uint32_t r;
+    asm("v_writelane_b32 %0, %1, %2" : "=v"(r) : "s"(val), "i"(L), "0"(old));
+    return r;
+}
+
+// words V.. of a chunk's masks into their lanes (compile-time lane numbers)
+template <int V>
+__device__ __forceinline__ void lane_masks(const uint64_t (&M)[VT], uint32_t& mlo, uint32_t& mhi, uint32_t& pre,
+                                           uint32_t& run) {
+    if constexpr (V < VT) {
+        mlo = write_lane<V>(mlo, uint32_t(M[V]));
+        mhi = write_lane<V>(mhi, uint32_t(M[V] >> 32));
+        pre = write_lane<V>(pre, run);
+        run += uint32_t(__popcll(M[V]));
+        lane_masks<V + 1>(M, mlo, mhi, pre, run);
+    } else {
+        pre = write_lane<VT>(pre, run);
+    }
+}
+
 // A wave's chunk of 64 x VT positions is probed into VT ballot masks (bit l
 // of word v: position 64 v + l).  The wave stores the masks and the hits
 // before each word (entry VT: no bits, the chunk's total) in its own LDS
@@ -94,16 +118,10 @@ __device__ __forceinline__ uint32_t wave_first(uint32_t lo, uint32_t hi, Pass pa
 // popcounts per lane instead of masking all VT words.
 __device__ __forceinline__ void store_chunk_masks(const uint64_t (&M)[VT], uint64_t* wm, uint32_t* wp) {
     const uint32_t lane = lane_id();
-    uint32_t mlo = 0, mhi = 0, pre = 0, run = 0;  // lane v takes word v
-#pragma unroll
-    for (int v = 0; v < VT; ++v) {
-        const bool mine = lane == uint32_t(v);
-        mlo = mine ? uint32_t(M[v]) : mlo;
-        mhi = mine ? uint32_t(M[v] >> 32) : mhi;
-        pre = mine ? run : pre;
-        run += uint32_t(__popcll(M[v]));
-    }
-    if (lane == uint32_t(VT)) pre = run;
+    // lane v takes word v: the masks and running counts are wave-uniform, so
+    // each goes into its lane with one v_writelane
+    uint32_t mlo = 0, mhi = 0, pre = 0, run = 0;
+    lane_masks<0>(M, mlo, mhi, pre, run);
     if (lane <= uint32_t(VT)) {
         wm[lane] = (uint64_t(mhi) << 32) | mlo;
         wp[lane] = pre;
