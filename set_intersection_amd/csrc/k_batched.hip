@@ -131,6 +131,13 @@ __device__ __forceinline__ void store_chunk_masks(const uint64_t (&M)[VT], uint6
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// lanes l of word v whose chunk position 64 v + l is at most `last`
+// (wave-uniform: scalar instructions)
+__device__ __forceinline__ uint64_t lanes_upto(uint32_t last, int v) {
+    const uint32_t b = uint32_t(v) * WAVE;
+    return last >= b + (WAVE - 1) ? ~0ull : last < b ? 0ull : (2ull << (last - b)) - 1ull;
+}
+
 __device__ __forceinline__ uint32_t chunk_hits_before(const uint64_t* wm, const uint32_t* wp, uint32_t x) {
     const uint32_t v = x >> 6, r = x & 63u;  // x in [0, 64 VT]
     return wp[v] + uint32_t(__popcll(wm[v] & ((1ull << r) - 1ull)));
@@ -236,7 +243,9 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
     __shared__ RowSmem<LDSW> sm;
     __shared__ uint64_t wmask[NW][VT + 1];  // store_chunk_masks, per wave
     __shared__ uint32_t wpre[NW][VT + 1];
-    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+    // the wave number is wave-uniform: held in a scalar register, so the
+    // wave's partner range and chunk bases are scalar too
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = __builtin_amdgcn_readfirstlane(tid / WAVE);
     if (tid == 0) {
         sm.row = EMPTY;
         sm.mode = EMPTY;
@@ -277,16 +286,28 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                 const uint32_t* cb = selem + c0;
                 const uint32_t last = uint32_t(c1 - c0) - 1;
                 if (mode == MODE_BITMAP) {
-                    // the common case, branch-free
-                    uint32_t x[VT];
+                    // the common case, branch-free: a scalar chunk base plus a
+                    // 32-bit byte offset per load, one bit-field extract and
+                    // compare per position; a partial chunk's end is a scalar
+                    // lane mask per word
+                    const bool whole = last == uint32_t(WAVE * VT - 1);
+                    const char* cbb = reinterpret_cast<const char*>(cb);
+                    const uint32_t l4 = 4u * last;
+                    uint32_t x[VT], wd[VT];
 #pragma unroll
                     for (int v = 0; v < VT; ++v) {
-                        const uint32_t q = uint32_t(v) * WAVE + lane;
-                        x[v] = cb[q < last ? q : last];
+                        const uint32_t q4 = 4u * (uint32_t(v) * WAVE + lane);
+                        x[v] = *reinterpret_cast<const uint32_t*>(cbb + (q4 < l4 ? q4 : l4));
                     }
 #pragma unroll
+                    for (int v = 0; v < VT; ++v) wd[v] = sm.tab[x[v] >> 5];
+#pragma unroll
                     for (int v = 0; v < VT; ++v)
-                        M[v] = __ballot(((sm.tab[x[v] >> 5] >> (x[v] & 31)) & 1u) && uint32_t(v) * WAVE + lane <= last);
+                        M[v] = __builtin_amdgcn_ballot_w64(__builtin_amdgcn_ubfe(wd[v], x[v], 1u) != 0u);
+                    if (!whole) {
+#pragma unroll
+                        for (int v = 0; v < VT; ++v) M[v] &= lanes_upto(last, v);
+                    }
                 } else if (mode == MODE_HASH) {
                     uint32_t x[VT];
 #pragma unroll
