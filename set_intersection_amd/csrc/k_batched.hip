@@ -55,6 +55,9 @@ constexpr int PR_NT = 256, PR_WORDS = 8192;    // 32 KiB LDS: hash tables of <= 
 constexpr uint32_t HASH_MAX = PR_WORDS / 2;    // largest row that fits a hash table
 constexpr int SEARCH_NS = 4096;                // samples cached for SEARCH rows
 constexpr int VT = 24;                         // elements per lane per chunk (24 loads in flight: 27.6 -> 24.7 ms for MISI 16384 Zipf)
+// zeroed elements after the reordered element array: a wave's whole chunk
+// (WAVE * VT positions) starting at any element stays inside the allocation
+constexpr uint32_t SELEM_PAD = WAVE * VT;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t x, uint32_t logTS) { return (x * 0x9E3779B1u) >> (32 - logTS); }
 
@@ -287,18 +290,22 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                 const uint32_t last = uint32_t(c1 - c0) - 1;
                 if (mode == MODE_BITMAP) {
                     // the common case, branch-free: a scalar chunk base plus a
-                    // 32-bit byte offset per load, one bit-field extract and
-                    // compare per position; a partial chunk's end is a scalar
-                    // lane mask per word
+                    // constant offset per load (positions past c1 read other
+                    // sets' elements or the zeroed padding, all inside the
+                    // universe and so inside the bitmap), one bit-field
+                    // extract and compare per position; a partial chunk's end
+                    // is a scalar lane mask per word
                     const bool whole = last == uint32_t(WAVE * VT - 1);
-                    const char* cbb = reinterpret_cast<const char*>(cb);
-                    const uint32_t l4 = 4u * last;
                     uint32_t x[VT], wd[VT];
+                    // lane byte offset + an immediate per word (instruction
+                    // offsets reach 4095 bytes: words 16.. use a second base)
+                    const uint32_t lane4 = 4u * lane;
+                    const char* cb0 = reinterpret_cast<const char*>(cb);
+                    const char* cb1 = cb0 + 16 * 4 * WAVE;
 #pragma unroll
-                    for (int v = 0; v < VT; ++v) {
-                        const uint32_t q4 = 4u * (uint32_t(v) * WAVE + lane);
-                        x[v] = *reinterpret_cast<const uint32_t*>(cbb + (q4 < l4 ? q4 : l4));
-                    }
+                    for (int v = 0; v < VT; ++v)
+                        x[v] = *reinterpret_cast<const uint32_t*>((v < 16 ? cb0 + v * 4 * WAVE : cb1 + (v - 16) * 4 * WAVE) +
+                                                                  lane4);
 #pragma unroll
                     for (int v = 0; v < VT; ++v) wd[v] = sm.tab[x[v] >> 5];
 #pragma unroll
@@ -932,11 +939,11 @@ static int build_plan(si_dataset* ds) {
     while (plan->kne < k && plan->ssize[plan->kne] > 0) ++plan->kne;
     SI_HIP(hipMalloc(&plan->d_perm, std::max<uint64_t>(k, 1) * 4));
     SI_HIP(hipMalloc(&plan->d_soff, (uint64_t(k) + 1) * 8));
-    SI_HIP(hipMalloc(&plan->d_selem, (ds->total + 64) * 4));
+    SI_HIP(hipMalloc(&plan->d_selem, (ds->total + SELEM_PAD) * 4));
     SI_HIP(hipMalloc(&plan->d_queue, 256));
     SI_HIP(hipMemcpyAsync(plan->d_perm, plan->perm.data(), uint64_t(k) * 4, hipMemcpyHostToDevice, ctx->stream));
     SI_HIP(hipMemcpyAsync(plan->d_soff, plan->soff.data(), (uint64_t(k) + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
-    SI_HIP(hipMemsetAsync(plan->d_selem + ds->total, 0, 64 * 4, ctx->stream));
+    SI_HIP(hipMemsetAsync(plan->d_selem + ds->total, 0, SELEM_PAD * 4, ctx->stream));
     SI_HIP(hipMemsetAsync(plan->d_queue, 0, 256, ctx->stream));
     if (k) {
         KernelScope ks(ctx, "batched_gather");
