@@ -344,9 +344,10 @@ int si_edge_count_async(si_edge_plan* plan, si_buffer* counts, si_buffer* total_
  * the search ratio; each candidate planned, run once to warm up and twice
  * timed, the faster run counting) until budget_ms of wall time is spent, and
  * returns the fastest record for si_edge_plan_create.  With tuning == NULL,
- * si_edge_plan_create uses the edge defaults: those of
- * si_batched_default_tuning with 16384 partner elements per hash-row item
- * and two probe workgroups per CU. */
+ * si_edge_plan_create uses the edge defaults: bitmap rows from 512
+ * elements, 2 Mi partner elements per bitmap-row item, 16384 per hash-row
+ * item and two probe workgroups per CU (the other fields as in
+ * si_batched_default_tuning). */
 int si_edge_autotune(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t m, double budget_ms,
                      si_tuning* out);
 /* The plain baseline of si_edge_count_async on the same plan and outputs:

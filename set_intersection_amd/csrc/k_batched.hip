@@ -890,17 +890,19 @@ static bool same_tuning(const si_tuning& a, const si_tuning& b) {
 static uint32_t bitmap_words(si_dataset* ds) { return uint32_t((uint64_t(ds->max_elem) >> 5) + 1); }
 
 // Defaults measured on MI355X (the autotuner's picks): the all-pairs
-// dispatcher on 16384 Zipf(1.2) sets (17.6 -> 16.3 ms against 64 Ki / 1 Mi
-// partner elements per item) and the edge dispatcher on R-MAT 20 (small
-// hash-row items, two probe workgroups per CU: 10.6 -> 9.5 ms).  bitmap_min
-// 512 is fastest for both.
+// dispatcher on 16384 Zipf(1.2) sets, every row with a bitmap in 8 Mi
+// partner elements per item (10.5 -> 9.4 ms against bitmap rows from 512
+// elements and 2 Mi per item, since a bitmap probe costs four VALU
+// instructions and a hash probe several times that), and the edge
+// dispatcher on R-MAT 20 (bitmap rows from 512 elements, small hash-row
+// items, two probe workgroups per CU: 10.6 -> 9.5 ms).
 int batched_default_tuning(si_dataset*, si_tuning* t) {
     std::memset(t, 0, sizeof(*t));
     t->version = SI_TUNING_VERSION;
-    t->bitmap_min = 512;
+    t->bitmap_min = 16;
     t->ratio_search = 32;
     t->chunk_elems = 262144;
-    t->chunk_elems_bitmap = 1u << 21;
+    t->chunk_elems_bitmap = 1u << 23;
     t->probe_wg_per_cu = 4;
     t->use_graph = 1;
     return SI_OK;
@@ -908,7 +910,9 @@ int batched_default_tuning(si_dataset*, si_tuning* t) {
 
 static void edge_default_tuning(si_dataset* ds, si_tuning* t) {
     batched_default_tuning(ds, t);
+    t->bitmap_min = 512;
     t->chunk_elems = 16384;
+    t->chunk_elems_bitmap = 1u << 21;
     t->probe_wg_per_cu = 2;
 }
 
@@ -1178,14 +1182,14 @@ int batched_autotune_seeded(si_dataset* ds, const si_tuning* seed, double budget
     // coordinate descent over the dispatcher's knobs: fixed candidate lists,
     // or values around the seed's when the descent starts from a seed
     std::vector<std::pair<uint32_t si_tuning::*, std::vector<uint32_t>>> knobs = {
-        {&si_tuning::bitmap_min, {512, 1024, 2048, 4096}},
+        {&si_tuning::bitmap_min, {16, 64, 256, 512, 1024, 2048, 4096}},
         {&si_tuning::chunk_elems, {16384, 32768, 65536, 131072, 262144}},
-        {&si_tuning::chunk_elems_bitmap, {262144, 524288, 1u << 20, 1u << 21}},
+        {&si_tuning::chunk_elems_bitmap, {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24}},
         {&si_tuning::probe_wg_per_cu, {2, 4, 6, 8}},
         {&si_tuning::ratio_search, {8, 16, 32, 64}},
     };
     if (seed) {
-        knobs[0].second = around(base.bitmap_min, 64, 1u << 16);
+        knobs[0].second = around(base.bitmap_min, 16, 1u << 16);
         knobs[1].second = around(base.chunk_elems, 4096, 1u << 22);
         knobs[2].second = around(base.chunk_elems_bitmap, 1u << 16, 1u << 24);
         knobs[3].second = {std::max(1u, base.probe_wg_per_cu / 2), base.probe_wg_per_cu, base.probe_wg_per_cu + 2,
@@ -1580,8 +1584,8 @@ int edge_autotune(si_dataset* ds, const uint32_t* u, const uint32_t* v, uint64_t
     int st = measure(*best, bestMs);
     // one pass of coordinate descent over the knobs that shape the edge plan
     const std::vector<std::pair<uint32_t si_tuning::*, std::vector<uint32_t>>> knobs = {
-        {&si_tuning::bitmap_min, {256, 1024, 2048}},
-        {&si_tuning::chunk_elems_bitmap, {1u << 19, 1u << 21}},
+        {&si_tuning::bitmap_min, {64, 256, 1024, 2048}},
+        {&si_tuning::chunk_elems_bitmap, {1u << 19, 1u << 23}},
         {&si_tuning::chunk_elems, {8192, 32768}},
         {&si_tuning::probe_wg_per_cu, {4, 8}},
         {&si_tuning::ratio_search, {16, 64}},
