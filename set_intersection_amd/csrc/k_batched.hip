@@ -284,10 +284,19 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
             for (uint64_t c0 = ws; c0 < we; c0 += WAVE * VT) {
                 const uint64_t c1 = (c0 + WAVE * VT < we) ? c0 + WAVE * VT : we;
                 uint64_t M[VT];
-                // chunk position q = 64 v + lane from a uniform chunk base and a
-                // 32-bit offset; positions past c1 re-read c1 - 1 (masked off)
                 const uint32_t* cb = selem + c0;
                 const uint32_t last = uint32_t(c1 - c0) - 1;
+                // bounds and output numbers of the sets jc + t, t < 2 x 64,
+                // loaded beside the chunk (sets past j1 read set j1 - 1's)
+                uint64_t pBeg[2], pEnd[2];
+                uint32_t pPerm[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t j = jc + uint32_t(h) * WAVE + lane, jj = j < I.j1 ? j : I.j1 - 1;
+                    pBeg[h] = soff[jj];
+                    pEnd[h] = soff[jj + 1];
+                    pPerm[h] = perm[jj];
+                }
                 if (mode == MODE_BITMAP) {
                     // the common case, branch-free: a scalar chunk base plus a
                     // constant offset per load (positions past c1 read other
@@ -316,6 +325,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                         for (int v = 0; v < VT; ++v) M[v] &= lanes_upto(last, v);
                     }
                 } else if (mode == MODE_HASH) {
+                    // positions past c1 re-read c1 - 1 (masked off)
                     uint32_t x[VT];
 #pragma unroll
                     for (int v = 0; v < VT; ++v) {
@@ -340,15 +350,24 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                 uint32_t nextJc = jc, nextCarry = 0;
                 for (uint32_t g = 0;; g += WAVE) {
                     const uint32_t j = jc + g + lane;
-                    const bool valid = j < I.j1 && soff[j] < c1;
+                    uint64_t sBeg, sEnd;
+                    uint32_t pj;
+                    if (g < 2 * WAVE) {
+                        sBeg = g ? pBeg[1] : pBeg[0];
+                        sEnd = g ? pEnd[1] : pEnd[0];
+                        pj = g ? pPerm[1] : pPerm[0];
+                    } else {
+                        const uint32_t jj = j < I.j1 ? j : I.j1 - 1;
+                        sBeg = soff[jj];
+                        sEnd = soff[jj + 1];
+                        pj = perm[jj];
+                    }
+                    const bool valid = j < I.j1 && sBeg < c1;
                     const uint64_t valid_mask = __ballot(valid);
                     if (!valid_mask) break;
                     uint32_t cnt = 0;
-                    uint64_t sEnd = 0;
                     bool endsHere = false;
                     if (valid) {
-                        const uint64_t sBeg = soff[j];
-                        sEnd = soff[j + 1];
                         const uint32_t a = uint32_t((sBeg > c0 ? sBeg : c0) - c0);
                         const uint32_t b = uint32_t((sEnd < c1 ? sEnd : c1) - c0);
                         cnt = chunk_hits_before(wmask[w], wpre[w], b) - chunk_hits_before(wmask[w], wpre[w], a);
@@ -356,7 +375,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                         endsHere = sEnd <= c1;
                         if (endsHere) {
                             if (sEnd <= we && sBeg >= ws) {
-                                out[pair_slot(k, prow, perm[j])] = cnt;
+                                out[pair_slot(k, prow, pj)] = cnt;
                             } else if (sEnd <= we) {
                                 sm.pset[2 * w] = j;  // started before this wave
                                 sm.pcnt[2 * w] = cnt;
@@ -368,9 +387,9 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                     }
                     // the last valid lane's set decides where the next chunk starts
                     const int last = 63 - __builtin_clzll(valid_mask);
-                    const uint32_t lastJ = __shfl(j, last, WAVE);
-                    const uint32_t lastEnds = __shfl(uint32_t(endsHere), last, WAVE);
-                    const uint32_t lastCnt = __shfl(cnt, last, WAVE);
+                    const uint32_t lastJ = jc + g + uint32_t(last);
+                    const uint32_t lastEnds = __builtin_amdgcn_readlane(uint32_t(endsHere), last);
+                    const uint32_t lastCnt = __builtin_amdgcn_readlane(cnt, last);
                     if (lastEnds) {
                         nextJc = lastJ + 1;
                         nextCarry = 0;
