@@ -281,6 +281,23 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
             // first set: largest j in [j0, j1) with soff[j] <= ws (soff[j0] = e0 <= ws)
             uint32_t jc = wave_first(I.j0 + 1, I.j1, [&](uint32_t j) { return soff[j] > ws; }) - 1;
             uint32_t carry = 0;
+            // bitmap rows load a chunk's elements while the previous chunk's
+            // sets are counted: a scalar chunk base plus a lane byte offset
+            // and an immediate per word (instruction offsets reach 4095
+            // bytes: words 16.. use a second base); positions past the
+            // wave's end read other sets' elements or the zeroed padding,
+            // all inside the universe and so inside the bitmap
+            const uint32_t lane4 = 4u * lane;
+            uint32_t xb[VT];
+            auto load_bitmap_chunk = [&](uint64_t c) {
+                const char* cb0 = reinterpret_cast<const char*>(selem + c);
+                const char* cb1 = cb0 + 16 * 4 * WAVE;
+#pragma unroll
+                for (int v = 0; v < VT; ++v)
+                    xb[v] = *reinterpret_cast<const uint32_t*>((v < 16 ? cb0 + v * 4 * WAVE : cb1 + (v - 16) * 4 * WAVE) +
+                                                               lane4);
+            };
+            if (mode == MODE_BITMAP) load_bitmap_chunk(ws);
             for (uint64_t c0 = ws; c0 < we; c0 += WAVE * VT) {
                 const uint64_t c1 = (c0 + WAVE * VT < we) ? c0 + WAVE * VT : we;
                 uint64_t M[VT];
@@ -298,32 +315,21 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
                     pPerm[h] = perm[jj];
                 }
                 if (mode == MODE_BITMAP) {
-                    // the common case, branch-free: a scalar chunk base plus a
-                    // constant offset per load (positions past c1 read other
-                    // sets' elements or the zeroed padding, all inside the
-                    // universe and so inside the bitmap), one bit-field
-                    // extract and compare per position; a partial chunk's end
-                    // is a scalar lane mask per word
+                    // the common case, branch-free: one bit-field extract and
+                    // compare per position; a partial chunk's end is a scalar
+                    // lane mask per word
                     const bool whole = last == uint32_t(WAVE * VT - 1);
-                    uint32_t x[VT], wd[VT];
-                    // lane byte offset + an immediate per word (instruction
-                    // offsets reach 4095 bytes: words 16.. use a second base)
-                    const uint32_t lane4 = 4u * lane;
-                    const char* cb0 = reinterpret_cast<const char*>(cb);
-                    const char* cb1 = cb0 + 16 * 4 * WAVE;
+                    uint32_t wd[VT];
+#pragma unroll
+                    for (int v = 0; v < VT; ++v) wd[v] = sm.tab[xb[v] >> 5];
 #pragma unroll
                     for (int v = 0; v < VT; ++v)
-                        x[v] = *reinterpret_cast<const uint32_t*>((v < 16 ? cb0 + v * 4 * WAVE : cb1 + (v - 16) * 4 * WAVE) +
-                                                                  lane4);
-#pragma unroll
-                    for (int v = 0; v < VT; ++v) wd[v] = sm.tab[x[v] >> 5];
-#pragma unroll
-                    for (int v = 0; v < VT; ++v)
-                        M[v] = __builtin_amdgcn_ballot_w64(__builtin_amdgcn_ubfe(wd[v], x[v], 1u) != 0u);
+                        M[v] = __builtin_amdgcn_ballot_w64(__builtin_amdgcn_ubfe(wd[v], xb[v], 1u) != 0u);
                     if (!whole) {
 #pragma unroll
                         for (int v = 0; v < VT; ++v) M[v] &= lanes_upto(last, v);
                     }
+                    if (c1 < we) load_bitmap_chunk(c1);
                 } else if (mode == MODE_HASH) {
                     // positions past c1 re-read c1 - 1 (masked off)
                     uint32_t x[VT];
