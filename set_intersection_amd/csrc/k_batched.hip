@@ -61,9 +61,12 @@ constexpr uint32_t SELEM_PAD = WAVE * VT;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t x, uint32_t logTS) { return (x * 0x9E3779B1u) >> (32 - logTS); }
 
+// tri_index(k, lo, hi) with 32-bit factors: one 32 x 32 -> 64-bit multiply
+// per lane (k(k-1)/2 is wave-uniform)
 __device__ __forceinline__ uint64_t pair_slot(uint32_t k, uint32_t pa, uint32_t pb) {
     const uint32_t lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
-    return tri_index(k, lo, hi);
+    const uint32_t m = k - lo;  // >= 2: lo < hi < k
+    return (uint64_t(k) * (k - 1) >> 1) - (uint64_t(m) * (m - 1) >> 1) + (hi - lo - 1);
 }
 
 // First index j in [lo, hi) with pass(j), or hi if none (pass monotone:
@@ -100,17 +103,18 @@ __device__ __forceinline__ uint32_t write_lane(uint32_t old, uint32_t val) {
 
 // words V.. of a chunk's masks into their lanes (compile-time lane numbers)
 template <int V>
-__device__ __forceinline__ void lane_masks(const uint64_t (&M)[VT], uint32_t& mlo, uint32_t& mhi, uint32_t& pre,
-                                           uint32_t& run) {
+__device__ __forceinline__ void lane_masks(const uint64_t (&M)[VT], uint32_t& mlo, uint32_t& mhi) {
     if constexpr (V < VT) {
         mlo = write_lane<V>(mlo, uint32_t(M[V]));
         mhi = write_lane<V>(mhi, uint32_t(M[V] >> 32));
-        pre = write_lane<V>(pre, run);
-        run += uint32_t(__popcll(M[V]));
-        lane_masks<V + 1>(M, mlo, mhi, pre, run);
-    } else {
-        pre = write_lane<VT>(pre, run);
+        lane_masks<V + 1>(M, mlo, mhi);
     }
+}
+
+// x + (x of lane l - d within the lane's row of 16; 0 for the row's first d lanes)
+template <int D>
+__device__ __forceinline__ uint32_t add_row_shr(uint32_t x) {
+    return x + uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x110 + D, 0xf, 0xf, false));
 }
 
 // A wave's chunk of 64 x VT positions is probed into VT ballot masks (bit l
@@ -120,14 +124,23 @@ __device__ __forceinline__ void lane_masks(const uint64_t (&M)[VT], uint32_t& ml
 // are chunk_hits_before(b) - chunk_hits_before(a): two LDS reads and two
 // popcounts per lane instead of masking all VT words.
 __device__ __forceinline__ void store_chunk_masks(const uint64_t (&M)[VT], uint64_t* wm, uint32_t* wp) {
+    static_assert(VT + 1 <= 32, "the prefix scan covers lanes 0..31");
     const uint32_t lane = lane_id();
-    // lane v takes word v: the masks and running counts are wave-uniform, so
-    // each goes into its lane with one v_writelane
-    uint32_t mlo = 0, mhi = 0, pre = 0, run = 0;
-    lane_masks<0>(M, mlo, mhi, pre, run);
+    // lane v takes word v (one v_writelane per half: the masks are
+    // wave-uniform); the hits before word v are an exclusive prefix sum of
+    // the words' popcounts over lanes 0..VT (lanes >= VT hold no bits):
+    // a scan within rows of 16 lanes, then row 0's total added to row 1
+    uint32_t mlo = 0, mhi = 0;
+    lane_masks<0>(M, mlo, mhi);
+    const uint32_t c = uint32_t(__builtin_popcount(mlo)) + uint32_t(__builtin_popcount(mhi));
+    uint32_t x = add_row_shr<1>(c);
+    x = add_row_shr<2>(x);
+    x = add_row_shr<4>(x);
+    x = add_row_shr<8>(x);
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xa, 0xf, false));  // row_bcast:15
     if (lane <= uint32_t(VT)) {
         wm[lane] = (uint64_t(mhi) << 32) | mlo;
-        wp[lane] = pre;
+        wp[lane] = x - c;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
