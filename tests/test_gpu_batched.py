@@ -52,6 +52,22 @@ def test_batched_big_rows_search_and_merge(ctx, ratio):
     assert np.array_equal(ds.batched_count(t), exp)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batched_bitmap_rows_many_tiny_partners(ctx, graph):
+    """Default tuning: two large bitmap rows whose partners are thousands of
+    sets of 1-3 elements, so one wave's chunk spans more than 128 sets (the
+    set bounds loaded beside a chunk cover 128; the rest are loaded in the
+    counting loop), plus partial chunks at every wave's end."""
+    rng = np.random.default_rng(11)
+    sizes = [20000, 15000, 600, 40] + list(rng.integers(1, 4, 3000))
+    coll = datasets.sized_collection(sizes, 1 << 16, 12)
+    ds = ctx.upload(coll)
+    exp, _ = oracle.all_pairs(coll)
+    t = tuning(ds, use_graph=graph)
+    assert np.array_equal(ds.batched_count(t), exp)
+    assert np.array_equal(ds.batched_count(t), exp)
+
+
 def test_batched_empty_and_tiny(ctx):
     coll = si.Collection.from_sets([[], [1, 2, 3], [], [2], [3, 4, 2**32 - 1], [2**32 - 1], []])
     ds = ctx.upload(coll)
