@@ -250,7 +250,8 @@ __device__ __forceinline__ uint32_t probe(const RowSmem<LDSW>& sm, uint32_t mode
 // Persistent row kernel: pops items from queue[0], builds the row's LDS
 // structure, streams the item's partner elements and stores one count per pair.
 template <int NT, int LDSW>
-__global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uint32_t nitems, uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, const uint32_t* __restrict__ starts,
+                                             uint32_t nitems, uint32_t* __restrict__ queue,
                                              const uint32_t* __restrict__ selem, const uint64_t* __restrict__ soff,
                                              const uint32_t* __restrict__ perm, uint32_t k, uint32_t bmWords,
                                              uint32_t* __restrict__ out) {
@@ -274,6 +275,9 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
         const uint32_t it = sm.item;
         if (it >= nitems) break;
         const Item I = items[it];
+        // this wave's first partner set: the largest j in [j0, j1) with
+        // soff[j] <= the wave's first partner element (from the plan)
+        const uint32_t jcFirst = starts[uint64_t(it) * NW + w];
         if (I.row != sm.row || I.mode != sm.mode) {
             __syncthreads();  // everyone has read sm.row / sm.mode
             build_row<NT, LDSW>(sm, selem + soff[I.row], soff[I.row + 1] - soff[I.row], I.mode, bmWords);
@@ -291,8 +295,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, uin
         const uint64_t e0 = soff[I.j0], e1 = soff[I.j1], len = e1 - e0;
         const uint64_t ws = e0 + (len * w) / NW, we = e0 + (len * (w + 1)) / NW;
         if (ws < we) {
-            // first set: largest j in [j0, j1) with soff[j] <= ws (soff[j0] = e0 <= ws)
-            uint32_t jc = wave_first(I.j0 + 1, I.j1, [&](uint32_t j) { return soff[j] > ws; }) - 1;
+            uint32_t jc = jcFirst;
             uint32_t carry = 0;
             // bitmap rows load a chunk's elements while the previous chunk's
             // sets are counted: a scalar chunk base plus a lane byte offset
@@ -900,9 +903,11 @@ struct BatchPlan {
     bool have_items = false;
     Item* d_bm_items = nullptr;
     Item* d_pr_items = nullptr;
+    uint32_t* d_bm_starts = nullptr;  // per item, each wave's first partner set
+    uint32_t* d_pr_starts = nullptr;
     uint4* d_merge = nullptr;
     uint32_t n_bm = 0, n_pr = 0, n_merge = 0;
-    uint64_t items_cap_bm = 0, items_cap_pr = 0, merge_cap = 0;
+    uint64_t items_cap_bm = 0, items_cap_pr = 0, merge_cap = 0, starts_cap_bm = 0, starts_cap_pr = 0;
     // captured launch sequence
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -912,7 +917,7 @@ struct BatchPlan {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         for (void* p : {(void*)d_perm, (void*)d_soff, (void*)d_selem, (void*)d_queue, (void*)d_bm_items,
-                        (void*)d_pr_items, (void*)d_merge})
+                        (void*)d_pr_items, (void*)d_bm_starts, (void*)d_pr_starts, (void*)d_merge})
             if (p) (void)hipFree(p);
     }
 };
@@ -1064,6 +1069,23 @@ static int build_items(si_dataset* ds, const si_tuning& t) {
         return out;
     };
     std::vector<Item> bm = chunkMajor(bmT), pr = chunkMajor(prT);
+    // each wave's first partner set: the largest j in [j0, j1) with
+    // soff[j] <= the wave's first element (the waves split an item's partner
+    // elements evenly, as the row kernel does)
+    auto starts = [&](const std::vector<Item>& items, uint32_t nw) {
+        std::vector<uint32_t> st(items.size() * nw);
+        for (size_t q = 0; q < items.size(); ++q) {
+            const Item& it = items[q];
+            const uint64_t e0 = P->soff[it.j0], len = P->soff[it.j1] - e0;
+            for (uint32_t w = 0; w < nw; ++w) {
+                const uint64_t ws = e0 + (len * w) / nw;
+                st[q * nw + w] = uint32_t(std::upper_bound(P->soff.begin() + it.j0 + 1, P->soff.begin() + it.j1, ws) -
+                                          P->soff.begin()) - 1;
+            }
+        }
+        return st;
+    };
+    std::vector<uint32_t> bmStarts = starts(bm, BM_NT / WAVE), prStarts = starts(pr, PR_NT / WAVE);
     auto upload = [&](auto& vec, auto*& dptr, uint64_t& cap) -> int {
         using T = typename std::decay<decltype(vec)>::type::value_type;
         if (vec.size() > cap || !dptr) {
@@ -1080,6 +1102,8 @@ static int build_items(si_dataset* ds, const si_tuning& t) {
         return fail(SI_ERR_RANGE, "too many work items");
     SI_TRY(upload(bm, P->d_bm_items, P->items_cap_bm));
     SI_TRY(upload(pr, P->d_pr_items, P->items_cap_pr));
+    SI_TRY(upload(bmStarts, P->d_bm_starts, P->starts_cap_bm));
+    SI_TRY(upload(prStarts, P->d_pr_starts, P->starts_cap_pr));
     SI_TRY(upload(merge, P->d_merge, P->merge_cap));
     SI_HIP(hipStreamSynchronize(ctx->stream));
     P->n_bm = uint32_t(bm.size());
@@ -1105,14 +1129,16 @@ static int enqueue(si_dataset* ds, const si_tuning& t, uint32_t* out) {
     if (P->n_bm) {
         KernelScope ks(ctx, "batched_rows_bitmap");
         const uint32_t grid = uint32_t(std::min<uint64_t>(ctx->num_cus, P->n_bm));
-        hipLaunchKernelGGL((k_rows<BM_NT, BM_WORDS>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_bm_items, P->n_bm,
-                           P->d_queue + 0, P->d_selem, P->d_soff, P->d_perm, k, bitmap_words(ds), out);
+        hipLaunchKernelGGL((k_rows<BM_NT, BM_WORDS>), dim3(grid), dim3(BM_NT), 0, ctx->stream, P->d_bm_items,
+                           P->d_bm_starts, P->n_bm, P->d_queue + 0, P->d_selem, P->d_soff, P->d_perm, k,
+                           bitmap_words(ds), out);
     }
     if (P->n_pr) {
         KernelScope ks(ctx, "batched_rows_probe");
         const uint32_t grid = uint32_t(std::min<uint64_t>(uint64_t(ctx->num_cus) * t.probe_wg_per_cu, P->n_pr));
-        hipLaunchKernelGGL((k_rows<PR_NT, PR_WORDS>), dim3(grid), dim3(PR_NT), 0, ctx->stream, P->d_pr_items, P->n_pr,
-                           P->d_queue + 1, P->d_selem, P->d_soff, P->d_perm, k, bitmap_words(ds), out);
+        hipLaunchKernelGGL((k_rows<PR_NT, PR_WORDS>), dim3(grid), dim3(PR_NT), 0, ctx->stream, P->d_pr_items,
+                           P->d_pr_starts, P->n_pr, P->d_queue + 1, P->d_selem, P->d_soff, P->d_perm, k,
+                           bitmap_words(ds), out);
     }
     if (P->n_merge) {
         KernelScope ks(ctx, "batched_merge_pairs");
