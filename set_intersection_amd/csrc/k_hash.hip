@@ -16,14 +16,15 @@
 //  * SISI is tiled by value: the universe is cut into buckets of 2^bits
 //    values (about 4096 elements of the larger set each) and a workgroup
 //    takes a run of consecutive buckets.  Per bucket it inserts the bucket's
-//    elements of the SMALLER set into an LDS table sized for them (groups of
-//    4 slots, at least twice as many slots as elements, at least 64) and
-//    probes it with the bucket's elements of the larger set; both sets
-//    stream once from HBM in order.  A key goes to the first empty slot of
-//    its group (the next group when it is full), so a probe reads one
-//    16-byte group and compares 4 keys; at load 1/2 about 86% of the groups
-//    have an empty slot.  A slot holds the value's low `bits` bits plus one,
-//    so 0 is the empty marker and clearing is a zero fill.  With --path the
+//    elements of the SMALLER set into an LDS table sized for them (16-byte
+//    groups of a fill counter and three key slots, at least eight slots per
+//    element where the table allows) and probes it with the bucket's
+//    elements of the larger set; both sets stream once from HBM in order.
+//    A key is the value's low `bits` bits (0xFFFFFFFF marks an empty slot).
+//    An insert claims a slot with one LDS atomic on its home group's counter
+//    (keys are unique, so no compare-and-swap) and passes on to the next
+//    group when the group is full; a probe reads one group and compares
+//    three keys, and only a full group sends it further.  With --path the
 //    pair is cut into merge-path segments instead (k_path.hip).
 //  * MISI runs keep the reference's ownership (workgroup, or one wave with
 //    --warp, per row a) and reduce each pair with wave shuffles.
@@ -82,7 +83,7 @@ __device__ __forceinline__ uint32_t log2_ceil(uint64_t v) {
 // pieces of TSMAX/2, each probed by all of the bucket's Q elements; the
 // bucket width keeps the expected S elements per bucket at 3072 or fewer.
 template <int NT, int U, int TSMAX>
-__global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict__ S, uint64_t nS,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_hash_buckets(const uint32_t* __restrict__ S, uint64_t nS,
                                                      const uint32_t* __restrict__ Q, uint64_t nQ, uint32_t bits,
                                                      uint64_t nb, uint32_t per, uint64_t* __restrict__ partials,
                                                      uint32_t* __restrict__ done, uint64_t* __restrict__ out64,
@@ -113,21 +114,29 @@ __global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict_
             while (k + 1 < nbd && (bnd[k] == bnd[k + 1] || bnd[nbd + k] == bnd[nbd + k + 1])) ++k;
             return k;
         };
+        // a bucket's elements of each side, NT x U at a time: a scalar base
+        // (the bucket's first element) and a 32-bit byte offset per load,
+        // clamped to the side's last element
         auto fetch = [&](uint32_t k, uint32_t (&xs)[U], uint32_t (&xq)[U]) {
             const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
+            const char* sb = reinterpret_cast<const char*>(S + uniform_u64(s0));
+            const char* qb = reinterpret_cast<const char*>(Q + uniform_u64(q0));
+            // a bucket holds at most 2^30 elements (bits <= 30): 4 (n - 1) fits 32 bits
+            const uint32_t sl = 4u * uniform_u32(uint32_t(s1 - s0) - 1u), ql = 4u * uniform_u32(uint32_t(q1 - q0) - 1u);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint64_t j = s0 + threadIdx.x + uint64_t(u) * NT;
-                xs[u] = S[j < s1 ? j : s1 - 1];
+                const uint32_t o = 4u * (threadIdx.x + uint32_t(u) * NT);
+                xs[u] = *reinterpret_cast<const uint32_t*>(sb + (o < sl ? o : sl));
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint64_t j = q0 + threadIdx.x + uint64_t(u) * NT;
-                xq[u] = Q[j < q1 ? j : q1 - 1];
+                const uint32_t o = 4u * (threadIdx.x + uint32_t(u) * NT);
+                xq[u] = *reinterpret_cast<const uint32_t*>(qb + (o < ql ? o : ql));
             }
         };
         constexpr uint64_t CAP = uint64_t(NT) * U;
         constexpr uint32_t PIECE = TSMAX / 2;
+        uint32_t hits = 0;  // this thread's matches from its registers (at most U per bucket)
         uint32_t xs[U], xq[U], ys[U], yq[U];
         uint32_t k = work(0);
         if (k + 1 < nbd) fetch(k, xs, xq);
@@ -137,93 +146,103 @@ __global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict_
             const uint32_t kn = work(k + 1);
             if (kn + 1 < nbd) fetch(kn, ys, yq);
             const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
+            const uint32_t qn = uniform_u32(uint32_t(q1 - q0 < CAP ? q1 - q0 : CAP));
             for (uint64_t p0 = s0; p0 < s1; p0 += PIECE) {
                 const uint32_t m = uint32_t(s1 - p0 < PIECE ? s1 - p0 : PIECE);
-                // 2^lg groups of 4 slots, at least 8m slots where TSMAX allows
-                // (load <= 1/8: most probes end in their first group), else
-                // 2m (m <= TSMAX / 2: load <= 1/2)
+                // 2^lg groups of 16 bytes: a fill counter and three key slots.
+                // At least 8m slots where TSMAX allows (most probes end in
+                // their first group), else 1.5 m (m <= TSMAX / 2)
                 constexpr uint32_t LGMAX = uint32_t(__builtin_ctz(uint32_t(TSMAX)));
                 uint32_t lg = 32u - uint32_t(__builtin_clz(8 * m - 1));
                 lg = lg < LGMAX ? lg : LGMAX;
                 lg = (lg < 6 ? 6 : lg) - 2;
                 const uint32_t gmask = (1u << lg) - 1u;
-                for (uint32_t i = threadIdx.x; i <= gmask; i += NT) tab4[i] = make_uint4(0u, 0u, 0u, 0u);
+                for (uint32_t i = threadIdx.x; i <= gmask; i += NT) tab4[i] = make_uint4(0u, EMPTY, EMPTY, EMPTY);
                 __syncthreads();
-                // S elements, four at a time side by side: a round reads each
-                // pending element's group (16 bytes), claims its first empty
-                // slot with one atomic, and moves to the next group when the
-                // group is full (a lost race re-reads the same group)
-                auto insert4 = [&](const uint32_t (&x)[4], uint32_t pend) {
-                    uint32_t key[4], g[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        key[j] = (x[j] & low) + 1u;
-                        g[j] = (key[j] * 0x9E3779B1u) >> (32 - lg);
-                    }
-                    while (pend) {
-                        uint4 t[4];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) t[j] = ((pend >> j) & 1u) ? tab4[g[j]] : make_uint4(1u, 1u, 1u, 1u);
-                        uint32_t prev[4];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const uint32_t w = t[j].x == 0u ? 0u : t[j].y == 0u ? 1u : t[j].z == 0u ? 2u : t[j].w == 0u ? 3u : 4u;
-                            const bool live = (pend >> j) & 1u;
-                            // keys are at most 2^30: ~0u marks a live element not placed this round
-                            prev[j] = (live && w < 4u) ? atomicCAS(&tab[4u * g[j] + w], 0u, key[j]) : (live ? ~0u : 0u);
-                            if (live && w == 4u) g[j] = (g[j] + 1u) & gmask;
+                // key: the value's low bits (below EMPTY); home group from a
+                // 24-bit multiplicative hash (full-rate multiply)
+                auto home = [&](uint32_t key) { return uint32_t(__umul24(key, 0x9E3779u)) >> (32 - lg); };  // HIP's __umul24 returns int
+                // claim a slot with one atomic on the group's counter; a full
+                // group passes the key on to the next (keys are unique)
+                auto insert = [&](uint32_t key, uint32_t g) {
+                    while (true) {
+                        const uint32_t c = atomicAdd(&tab[4u * g], 1u);
+                        if (c < 3u) {
+                            tab[4u * g + 1u + c] = key;
+                            return;
                         }
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (prev[j] == 0u || prev[j] == key[j]) pend &= ~(1u << j);
+                        g = (g + 1u) & gmask;
                     }
                 };
                 if (p0 == s0) {
+                    // the first piece from registers: every element's first
+                    // claim, then the stores (a claim past slot 3 walks on)
 #pragma unroll
-                    for (int b = 0; b < U; b += 4) {
-                        uint32_t pend = 0;
+                    for (int b = 0; b < U; b += 8) {
+                        if (uint32_t(b) * NT >= m) break;
+                        uint32_t g[8], c[8];
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) pend |= uint32_t(threadIdx.x + uint32_t(b + j) * NT < m) << j;
-                        if (pend) insert4({xs[b], xs[b + 1], xs[b + 2], xs[b + 3]}, pend);
-                    }
-                } else {
-                    for (uint64_t i = p0 + threadIdx.x; i < p0 + m; i += NT) insert4({S[i], 0u, 0u, 0u}, 1u);
-                }
-                __syncthreads();
-                // probes, four at a time: a key is present if its group or a
-                // following full group holds it; a group with an empty slot ends
-                // the search
-                auto probe4 = [&](const uint32_t (&x)[4], uint32_t pend) {
-                    uint32_t key[4], g[4], hits = 0;
+                        for (int j = 0; j < 8; ++j) {
+                            const bool v = threadIdx.x + uint32_t(b + j) * NT < m;
+                            g[j] = home(xs[b + j] & low);
+                            c[j] = v ? atomicAdd(&tab[4u * g[j]], 1u) : 3u;
+                        }
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        key[j] = (x[j] & low) + 1u;
-                        g[j] = (key[j] * 0x9E3779B1u) >> (32 - lg);
-                    }
-                    while (pend) {
-                        uint4 t[4];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) t[j] = ((pend >> j) & 1u) ? tab4[g[j]] : make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const bool live = (pend >> j) & 1u;
-                            const bool hit = t[j].x == key[j] || t[j].y == key[j] || t[j].z == key[j] || t[j].w == key[j];
-                            const bool full = t[j].x && t[j].y && t[j].z && t[j].w;
-                            hits += uint32_t(live && hit);
-                            if (live && (hit || !full)) pend &= ~(1u << j);
-                            g[j] = (live && !hit && full) ? ((g[j] + 1u) & gmask) : g[j];
+                        for (int j = 0; j < 8; ++j) {
+                            const bool v = threadIdx.x + uint32_t(b + j) * NT < m;
+                            if (v && c[j] < 3u) tab[4u * g[j] + 1u + c[j]] = xs[b + j] & low;
+                            else if (v) insert(xs[b + j] & low, (g[j] + 1u) & gmask);
                         }
                     }
-                    return hits;
+                } else {
+                    for (uint64_t i = p0 + threadIdx.x; i < p0 + m; i += NT) {
+                        const uint32_t key = S[i] & low;
+                        insert(key, home(key));
+                    }
+                }
+                __syncthreads();
+                // a key is present if its group, or a full group after it
+                // up to the first group that is not full, holds it
+                auto probe_rest = [&](uint32_t key, uint32_t g) {
+                    while (true) {
+                        g = (g + 1u) & gmask;
+                        const uint4 t = tab4[g];
+                        if (t.y == key || t.z == key || t.w == key) return true;
+                        if (t.x < 3u) return false;
+                    }
                 };
+                // four probes at a time: their group reads are in flight together
 #pragma unroll
                 for (int b = 0; b < U; b += 4) {
-                    uint32_t pend = 0;
+                    if (uint32_t(b) * NT >= qn) break;
+                    uint32_t key[4], g[4];
+                    uint4 t[4];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) pend |= uint32_t(q0 + threadIdx.x + uint64_t(b + j) * NT < q1) << j;
-                    if (pend) c += probe4({xq[b], xq[b + 1], xq[b + 2], xq[b + 3]}, pend);
+                    for (int j = 0; j < 4; ++j) {
+                        key[j] = xq[b + j] & low;
+                        g[j] = home(key[j]);
+                        t[j] = tab4[g[j]];
+                    }
+                    bool more[4], anyMore = false;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool v = threadIdx.x + uint32_t(b + j) * NT < qn;
+                        const bool hit = v && (t[j].y == key[j] || t[j].z == key[j] || t[j].w == key[j]);
+                        hits += hit ? 1u : 0u;
+                        more[j] = v && !hit && t[j].x >= 3u;
+                        anyMore = anyMore || more[j];
+                    }
+                    if (__builtin_amdgcn_ballot_w64(anyMore)) {  // rare: a full home group
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (more[j] && probe_rest(key[j], g[j])) ++hits;
+                    }
                 }
-                for (uint64_t i = q0 + CAP + threadIdx.x; i < q1; i += NT) c += probe4({Q[i], 0u, 0u, 0u}, 1u);
+                for (uint64_t i = q0 + CAP + threadIdx.x; i < q1; i += NT) {
+                    const uint32_t key = Q[i] & low, g = home(key);
+                    const uint4 t = tab4[g];
+                    c += (t.y == key || t.z == key || t.w == key || (t.x >= 3u && probe_rest(key, g))) ? 1u : 0u;
+                }
                 __syncthreads();  // probes done before the table is cleared again
             }
 #pragma unroll
@@ -233,6 +252,7 @@ __global__ __launch_bounds__(NT) void k_hash_buckets(const uint32_t* __restrict_
             }
             k = kn;
         }
+        c += hits;
     }
     if (prio) set_wave_prio(0);
     grid_fold_u64<NT>(c, partials, done, out64, out32, red);

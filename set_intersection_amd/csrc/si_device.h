@@ -12,6 +12,12 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// a wave-uniform value into a scalar register (first active lane's copy)
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return uint32_t(__builtin_amdgcn_readfirstlane(int(v))); }
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    return (uint64_t(uniform_u32(uint32_t(v >> 32))) << 32) | uniform_u32(uint32_t(v));
+}
+
 // Wave issue priority (s_setprio takes an immediate).  Kernels that stream
 // one long range per workgroup, several workgroups per CU, rotate it per
 // tile or bucket (priority (step + seed) mod 4): a CU otherwise issues from
