@@ -178,17 +178,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     // the first piece from registers: every element's first
                     // claim, then the stores (a claim past slot 3 walks on)
 #pragma unroll
-                    for (int b = 0; b < U; b += 8) {
+                    for (int b = 0; b < U; b += 4) {
                         if (uint32_t(b) * NT >= m) break;
-                        uint32_t g[8], c[8];
+                        uint32_t g[4], c[4];
 #pragma unroll
-                        for (int j = 0; j < 8; ++j) {
+                        for (int j = 0; j < 4; ++j) {
                             const bool v = threadIdx.x + uint32_t(b + j) * NT < m;
                             g[j] = home(xs[b + j] & low);
                             c[j] = v ? atomicAdd(&tab[4u * g[j]], 1u) : 3u;
                         }
 #pragma unroll
-                        for (int j = 0; j < 8; ++j) {
+                        for (int j = 0; j < 4; ++j) {
                             const bool v = threadIdx.x + uint32_t(b + j) * NT < m;
                             if (v && c[j] < 3u) tab[4u * g[j] + 1u + c[j]] = xs[b + j] & low;
                             else if (v) insert(xs[b + j] & low, (g[j] + 1u) & gmask);
