@@ -307,38 +307,50 @@ __device__ __forceinline__ uint32_t sample_search_multi(const uint32_t* __restri
 #pragma unroll
     for (int r = 0; r < R; ++r)
         if (lo[r] < NS && cache[lo[r]] <= x[r]) lo[r] += 1;
-    uint32_t hit = 0, active = 0;
-    uint64_t l[R], h[R];
+    // phase 2: the lower bound of x in S[l, h), where S[(j-1)*nS/NS] < x <
+    // S[j*nS/NS] (or the end of S), by a branch-free search of a fixed
+    // number of steps for every key: 32-bit positions within the interval
+    // (its width is at most ceil(nS / NS)), one load per key and step, the
+    // loads of the R keys independent
+    uint32_t hit = 0;
+    const uint32_t* B[R];
+    uint32_t n[R], pos[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        l[r] = h[r] = 0;
+        B[r] = S;
+        n[r] = 0;
+        pos[r] = 0;
         if (!valid[r] || lo[r] == 0) continue;  // x < S[0]
         if (cache[lo[r] - 1] == x[r]) {
             hit |= 1u << r;
             continue;
         }
-        // phase 2: S[(j-1)*nS/NS] < x < S[j*nS/NS]  (or the end of S)
-        l[r] = (uint64_t(lo[r] - 1) * nS) / NS + 1;
-        h[r] = (lo[r] == NS) ? nS : (uint64_t(lo[r]) * nS) / NS;
-        if (l[r] < h[r]) active |= 1u << r;
+        const uint64_t l = (uint64_t(lo[r] - 1) * nS) / NS + 1;
+        const uint64_t h = (lo[r] == NS) ? nS : (uint64_t(lo[r]) * nS) / NS;
+        if (l < h) {
+            B[r] = S + l;
+            n[r] = uint32_t(h - l);
+        }
     }
-    while (active) {
+    const uint64_t W = (nS + NS - 1) / NS;  // widest interval
+    const uint32_t T = W ? 64u - uint32_t(__builtin_clzll(W)) : 0u;  // 2^T > W
+    for (uint32_t k = T; k-- > 0;) {
+        const uint32_t step = 1u << k;
         uint32_t y[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            if ((active >> r) & 1u) y[r] = S[(l[r] + h[r]) >> 1];
-#pragma unroll
         for (int r = 0; r < R; ++r) {
-            if (!((active >> r) & 1u)) continue;
-            const uint64_t mid = (l[r] + h[r]) >> 1;
-            if (y[r] == x[r]) {
-                hit |= 1u << r;
-                active &= ~(1u << r);
-                continue;
-            }
-            if (y[r] < x[r]) l[r] = mid + 1; else h[r] = mid;
-            if (l[r] >= h[r]) active &= ~(1u << r);
+            // position pos + step - 1, clamped into [0, n) (a key with n = 0 reads B[0] = S[0])
+            const uint32_t q = pos[r] + step;
+            y[r] = B[r][(q <= n[r] ? q : (n[r] ? n[r] : 1u)) - 1u];
         }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (pos[r] + step <= n[r] && y[r] < x[r]) pos[r] += step;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t y = B[r][pos[r] < n[r] ? pos[r] : 0u];
+        if (pos[r] < n[r] && y == x[r]) hit |= 1u << r;
     }
     return hit;
 }
