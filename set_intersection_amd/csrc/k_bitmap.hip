@@ -81,20 +81,28 @@ __global__ __launch_bounds__(NT) void k_bm_buckets(const uint32_t* __restrict__ 
             while (k + 1 < nbd && (bnd[k] == bnd[k + 1] || bnd[nbd + k] == bnd[nbd + k + 1])) ++k;
             return k;
         };
+        // a scalar base (the bucket's first element) and a 32-bit byte
+        // offset per load, clamped to the side's last element (a bucket holds
+        // at most 2^30 elements, bits <= 30)
         auto fetch = [&](uint32_t k, uint32_t (&xs)[U], uint32_t (&xq)[U]) {
             const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
+            const char* sb = reinterpret_cast<const char*>(S + uniform_u64(s0));
+            const char* qb = reinterpret_cast<const char*>(Q + uniform_u64(q0));
+            const uint32_t sl = 4u * uniform_u32(uint32_t(s1 - s0) - 1u), ql = 4u * uniform_u32(uint32_t(q1 - q0) - 1u);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint64_t j = s0 + threadIdx.x + uint64_t(u) * NT;
-                xs[u] = S[j < s1 ? j : s1 - 1];
+                const uint32_t o = 4u * (threadIdx.x + uint32_t(u) * NT);
+                xs[u] = *reinterpret_cast<const uint32_t*>(sb + (o < sl ? o : sl));
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint64_t j = q0 + threadIdx.x + uint64_t(u) * NT;
-                xq[u] = Q[j < q1 ? j : q1 - 1];
+                const uint32_t o = 4u * (threadIdx.x + uint32_t(u) * NT);
+                xq[u] = *reinterpret_cast<const uint32_t*>(qb + (o < ql ? o : ql));
             }
         };
         constexpr uint64_t CAP = uint64_t(NT) * U;
+        const uint32_t wb = bits - 5;  // word index of a value: bits [5, bits)
+        uint32_t hits = 0;             // this thread's matches from its registers
         uint32_t xs[U], xq[U], ys[U], yq[U];
         uint32_t k = work(0);
         if (k + 1 < nbd) fetch(k, xs, xq);
@@ -106,9 +114,12 @@ __global__ __launch_bounds__(NT) void k_bm_buckets(const uint32_t* __restrict__ 
             const uint64_t s0 = bnd[k], s1 = bnd[k + 1], q0 = bnd[nbd + k], q1 = bnd[nbd + k + 1];
             for (uint32_t i = threadIdx.x; i < words / 4; i += NT) dyn[i] = make_uint4(0u, 0u, 0u, 0u);
             __syncthreads();
+            const uint32_t sn = uniform_u32(uint32_t(s1 - s0 < CAP ? s1 - s0 : CAP));
+            const uint32_t qn = uniform_u32(uint32_t(q1 - q0 < CAP ? q1 - q0 : CAP));
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (s0 + threadIdx.x + uint64_t(u) * NT < s1) atomicOr(&bm[(xs[u] & low) >> 5], 1u << (xs[u] & 31));
+                if (threadIdx.x + uint32_t(u) * NT < sn)
+                    atomicOr(&bm[__builtin_amdgcn_ubfe(xs[u], 5u, wb)], 1u << (xs[u] & 31));
             for (uint64_t i = s0 + CAP + threadIdx.x; i < s1; i += NT) {  // beyond NT * U: a dense bucket
                 const uint32_t x = S[i];
                 atomicOr(&bm[(x & low) >> 5], 1u << (x & 31));
@@ -116,7 +127,8 @@ __global__ __launch_bounds__(NT) void k_bm_buckets(const uint32_t* __restrict__ 
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                c += (q0 + threadIdx.x + uint64_t(u) * NT < q1) ? (bm[(xq[u] & low) >> 5] >> (xq[u] & 31)) & 1u : 0u;
+                hits += (threadIdx.x + uint32_t(u) * NT < qn) ? __builtin_amdgcn_ubfe(bm[__builtin_amdgcn_ubfe(xq[u], 5u, wb)], xq[u], 1u)
+                                                               : 0u;
             for (uint64_t i = q0 + CAP + threadIdx.x; i < q1; i += NT) {
                 const uint32_t x = Q[i];
                 c += (bm[(x & low) >> 5] >> (x & 31)) & 1u;
@@ -129,6 +141,7 @@ __global__ __launch_bounds__(NT) void k_bm_buckets(const uint32_t* __restrict__ 
             }
             k = kn;
         }
+        c += hits;
     }
     if (prio) set_wave_prio(0);
     grid_fold_u64<NT>(c, partials, done, out64, out32, red);
