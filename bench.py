@@ -280,25 +280,33 @@ def bench_misi(args, si, datasets, cpu_rows=4096):
     exp, _ = oracle.pairs(coll, a, b)
     idx = (k * (k - 1) // 2) - (k - a.astype(np.int64)) * (k - a.astype(np.int64) - 1) // 2 + b - a - 1
     exact = bool(np.array_equal(res, res2) and np.array_equal(res[idx], exp))
-    # plain per-technique kernels: the reference driver's runs with partition 10000
+    # plain per-technique kernels: the reference driver's runs with partition
+    # 10000, every technique and ownership (block / warp) the drivers offer
     plain = {}
     partition = 10000
     tiles = [(t, t * partition, min((t + 1) * partition, k)) for t in range((k + partition - 1) // partition)]
     runbuf = ctx.buffer(partition * partition * 4)
     for name, tech, warp in [("bitmap_dynamic", si.TECH_BITMAP_DYNAMIC, False), ("hi_block", si.TECH_HI, False),
-                             ("obs_block", si.TECH_OBS, False), ("ip", si.TECH_IP, False)]:
+                             ("obs_block", si.TECH_OBS, False), ("ip", si.TECH_IP, False),
+                             ("obs_warp", si.TECH_OBS, True), ("hi_warp", si.TECH_HI, True),
+                             ("bitmap_naive", si.TECH_BITMAP_NAIVE, False)]:
         if args.skip_plain and name != "bitmap_dynamic":
             continue
         p = si.params(warp=warp)
         ctx.sync()
         e0 = ctx.record()
-        for x in range(len(tiles)):
-            for y in range(x, len(tiles)):
-                ds.run_count_async(tech, si.tile(*tiles[x]), si.tile(*tiles[y]), partition, runbuf, p)
+        if tech == si.TECH_BITMAP_NAIVE:  # all pairs at once (bitmap_naive.cu's driver has no runs)
+            ds.all_pairs_bitmap_async(out)
+        else:
+            for x in range(len(tiles)):
+                for y in range(x, len(tiles)):
+                    ds.run_count_async(tech, si.tile(*tiles[x]), si.tile(*tiles[y]), partition, runbuf, p)
         e1 = ctx.record()
         ctx.sync()
         pms = ctx.elapsed_ms(e0, e1)
         plain[name] = {"ms": round(pms, 2), "pairs_per_s": round(pairs / (pms / 1e3), 1)}
+        if tech == si.TECH_BITMAP_NAIVE:  # same output layout as the dispatcher's
+            plain[name]["exact"] = bool(np.array_equal(out.download(np.uint32), res))
         log(f"[bench] plain {name}: {pms:.1f} ms")
         if pms > args.plain_limit_ms:
             log("[bench] skipping slower plain kernels")

@@ -276,6 +276,11 @@ class Dataset:
     def run_count_async(self, technique, A, B, n, counts, p=None):
         _check(_abi.lib().si_run_count_async(self.h, technique, byref(p) if p else None, A, B, n, counts.h))
 
+    def all_pairs_bitmap_async(self, counts, p=None):
+        """si_all_pairs_bitmap_async: every pair's count into `counts`
+        (uint32, triangular order) by bitmap AND + popcount."""
+        _check(_abi.lib().si_all_pairs_bitmap_async(self.h, byref(p) if p else None, counts.h))
+
     def all_pairs_bitmap(self, p=None):
         k = self.k
         counts = np.zeros(max(k * (k - 1) // 2, 1), dtype=np.uint32)
