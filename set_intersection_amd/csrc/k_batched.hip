@@ -330,6 +330,66 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, con
                     pEnd[h] = soff[jj + 1];
                     pPerm[h] = perm[jj];
                 }
+                // the chunk's counts from its masks: called at the end of each
+                // mode's branch, so the waits in the counting are those of its
+                // own path
+                auto count_sets = [&](const uint64_t (&M)[VT]) {
+                    store_chunk_masks(M, wmask[w], wpre[w]);
+                    // sets overlapping [c0, c1): jc, jc+1, ...; lane t of group g
+                    // takes jc + g + t.  A group returns whether the sets go on
+                    // past it.  The first two groups use the bounds loaded beside
+                    // the chunk (straight-line code, so their uses wait only for
+                    // those loads, not for the next chunk's, which are younger);
+                    // further groups load their own.
+                    uint32_t nextJc = jc, nextCarry = 0;
+                    auto group = [&](uint32_t g, uint64_t sBeg, uint64_t sEnd, uint32_t pj) -> bool {
+                        const uint32_t j = jc + g + lane;
+                        const bool valid = j < I.j1 && sBeg < c1;
+                        const uint64_t valid_mask = __ballot(valid);
+                        if (!valid_mask) return false;
+                        uint32_t cnt = 0;
+                        bool endsHere = false;
+                        if (valid) {
+                            const uint32_t a = uint32_t((sBeg > c0 ? sBeg : c0) - c0);
+                            const uint32_t b = uint32_t((sEnd < c1 ? sEnd : c1) - c0);
+                            cnt = chunk_hits_before(wmask[w], wpre[w], b) - chunk_hits_before(wmask[w], wpre[w], a);
+                            if (g == 0 && lane == 0) cnt += carry;
+                            endsHere = sEnd <= c1;
+                            if (endsHere) {
+                                if (sEnd <= we && sBeg >= ws) {
+                                    out[pair_slot(k, prow, pj)] = cnt;
+                                } else if (sEnd <= we) {
+                                    sm.pset[2 * w] = j;  // started before this wave
+                                    sm.pcnt[2 * w] = cnt;
+                                }
+                            } else if (c1 == we) {
+                                sm.pset[2 * w + 1] = j;  // continues past this wave
+                                sm.pcnt[2 * w + 1] = cnt;
+                            }
+                        }
+                        // the last valid lane's set decides where the next chunk starts
+                        const int last = 63 - __builtin_clzll(valid_mask);
+                        const uint32_t lastJ = jc + g + uint32_t(last);
+                        const uint32_t lastEnds = __builtin_amdgcn_readlane(uint32_t(endsHere), last);
+                        const uint32_t lastCnt = __builtin_amdgcn_readlane(cnt, last);
+                        if (lastEnds) {
+                            nextJc = lastJ + 1;
+                            nextCarry = 0;
+                        } else {
+                            nextJc = lastJ;
+                            nextCarry = lastCnt;
+                        }
+                        return last == 63;
+                    };
+                    if (group(0, pBeg[0], pEnd[0], pPerm[0]) && group(WAVE, pBeg[1], pEnd[1], pPerm[1])) {
+                        for (uint32_t g = 2 * WAVE;; g += WAVE) {
+                            const uint32_t j = jc + g + lane, jj = j < I.j1 ? j : I.j1 - 1;
+                            if (!group(g, soff[jj], soff[jj + 1], perm[jj])) break;
+                        }
+                    }
+                    jc = nextJc;
+                    carry = nextCarry;
+                };
                 if (mode == MODE_BITMAP) {
                     // the common case, branch-free: one bit-field extract and
                     // compare per position; a partial chunk's end is a scalar
@@ -345,7 +405,12 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, con
 #pragma unroll
                         for (int v = 0; v < VT; ++v) M[v] &= lanes_upto(last, v);
                     }
-                    if (c1 < we) load_bitmap_chunk(c1);
+                    // unconditional, so the counting below sees the same loads in
+                    // flight on every path (a conditional issue makes every wait
+                    // in it a wait for these loads too); the wave's last chunk
+                    // re-reads its own positions, which are cached
+                    load_bitmap_chunk(c1 < we ? c1 : c0);
+                    count_sets(M);
                 } else if (mode == MODE_HASH) {
                     // positions past c1 re-read c1 - 1 (masked off)
                     uint32_t x[VT];
@@ -358,6 +423,7 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, con
                     for (int v = 0; v < VT; ++v)
                         M[v] = __ballot(uint32_t(v) * WAVE + lane <= last &&
                                         probe<LDSW>(sm, MODE_HASH, x[v], S, rowN, logTS, hasMax));
+                    count_sets(M);
                 } else {
 #pragma unroll
                     for (int v = 0; v < VT; ++v) {
@@ -366,63 +432,8 @@ __global__ __launch_bounds__(NT) void k_rows(const Item* __restrict__ items, con
                         if (p < c1) hit = probe<LDSW>(sm, mode, selem[p], S, rowN, logTS, hasMax);
                         M[v] = __ballot(hit);
                     }
+                    count_sets(M);
                 }
-                store_chunk_masks(M, wmask[w], wpre[w]);
-                // sets overlapping [c0, c1): jc, jc+1, ...; lane t takes jc + g*64 + t
-                uint32_t nextJc = jc, nextCarry = 0;
-                for (uint32_t g = 0;; g += WAVE) {
-                    const uint32_t j = jc + g + lane;
-                    uint64_t sBeg, sEnd;
-                    uint32_t pj;
-                    if (g < 2 * WAVE) {
-                        sBeg = g ? pBeg[1] : pBeg[0];
-                        sEnd = g ? pEnd[1] : pEnd[0];
-                        pj = g ? pPerm[1] : pPerm[0];
-                    } else {
-                        const uint32_t jj = j < I.j1 ? j : I.j1 - 1;
-                        sBeg = soff[jj];
-                        sEnd = soff[jj + 1];
-                        pj = perm[jj];
-                    }
-                    const bool valid = j < I.j1 && sBeg < c1;
-                    const uint64_t valid_mask = __ballot(valid);
-                    if (!valid_mask) break;
-                    uint32_t cnt = 0;
-                    bool endsHere = false;
-                    if (valid) {
-                        const uint32_t a = uint32_t((sBeg > c0 ? sBeg : c0) - c0);
-                        const uint32_t b = uint32_t((sEnd < c1 ? sEnd : c1) - c0);
-                        cnt = chunk_hits_before(wmask[w], wpre[w], b) - chunk_hits_before(wmask[w], wpre[w], a);
-                        if (g == 0 && lane == 0) cnt += carry;
-                        endsHere = sEnd <= c1;
-                        if (endsHere) {
-                            if (sEnd <= we && sBeg >= ws) {
-                                out[pair_slot(k, prow, pj)] = cnt;
-                            } else if (sEnd <= we) {
-                                sm.pset[2 * w] = j;  // started before this wave
-                                sm.pcnt[2 * w] = cnt;
-                            }
-                        } else if (c1 == we) {
-                            sm.pset[2 * w + 1] = j;  // continues past this wave
-                            sm.pcnt[2 * w + 1] = cnt;
-                        }
-                    }
-                    // the last valid lane's set decides where the next chunk starts
-                    const int last = 63 - __builtin_clzll(valid_mask);
-                    const uint32_t lastJ = jc + g + uint32_t(last);
-                    const uint32_t lastEnds = __builtin_amdgcn_readlane(uint32_t(endsHere), last);
-                    const uint32_t lastCnt = __builtin_amdgcn_readlane(cnt, last);
-                    if (lastEnds) {
-                        nextJc = lastJ + 1;
-                        nextCarry = 0;
-                    } else {
-                        nextJc = lastJ;
-                        nextCarry = lastCnt;
-                    }
-                    if (last < 63) break;
-                }
-                jc = nextJc;
-                carry = nextCarry;
             }
         }
         __syncthreads();
