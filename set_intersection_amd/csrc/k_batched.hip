@@ -595,18 +595,18 @@ __device__ __forceinline__ void edge_batches(const uint4* __restrict__ part, uin
     }
 }
 
-// A wave's share [ws, we) of an item's concatenated partner lists, partner
-// by partner: the records of 64 partners at a time are loaded one per lane
-// (no dependent record load per partner), then the 64 lanes stream each
-// partner's elements (U loads in flight per lane) and probe them; a
-// partner's count is a sum of ballot popcounts.  A partner cut by ws or we
-// leaves its partial count in cutSet / cutCnt (slot 0: the one that begins
-// before ws, which is also the one when the whole share lies inside a single
-// partner; slot 1: the one that continues past we).  Partners of big rows
-// are long (hundreds of elements on average), so whole-wave runs waste few
-// lanes and need no position-to-partner search.  ADD: whole partners' counts
-// are added to out (a partner counted in two universe halves) instead of
-// stored.
+// A wave's share [ws, we) of an item's concatenated partner lists, two
+// partners at a time: the records of 64 partners at a time are loaded one per
+// lane (no dependent record load per partner), then the 64 lanes stream the
+// elements of two consecutive partners together (U loads in flight per lane
+// and partner) and probe them; a partner's count is a sum of ballot
+// popcounts.  A partner cut by ws or we leaves its partial count in cutSet /
+// cutCnt (slot 0: the one that begins before ws, which is also the one when
+// the whole share lies inside a single partner; slot 1: the one that
+// continues past we).  Partners of big rows are long (hundreds of elements on
+// average), so whole-wave runs waste few lanes and need no
+// position-to-partner search.  ADD: whole partners' counts are added to out
+// (a partner counted in two universe halves) instead of stored.
 template <int U, bool ADD, typename ProbeFn>
 __device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32_t j0, uint32_t j1, uint32_t ws,
                                           uint32_t we, const uint32_t* __restrict__ elems, ProbeFn probeFn,
@@ -619,39 +619,64 @@ __device__ __forceinline__ void edge_runs(const uint4* __restrict__ part, uint32
         const bool pv = qb + lane < j1;
         const uint4 R = pv ? part[qb + lane] : make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
         const uint32_t nrec = j1 - qb < uint32_t(WAVE) ? j1 - qb : uint32_t(WAVE);
-        uint32_t start = prevEnd;
-        for (uint32_t r = 0; r < nrec && start < we; ++r) {
-            const uint32_t end = uint32_t(__builtin_amdgcn_readlane(int(R.z), int(r)));
+        auto elemsOf = [&](uint32_t r) {
             const uint32_t offLo = uint32_t(__builtin_amdgcn_readlane(int(R.x), int(r)));
             const uint32_t offHi = uint32_t(__builtin_amdgcn_readlane(int(R.y), int(r)));
-            const uint32_t a = start > ws ? start : ws, b = end < we ? end : we;
-            const uint32_t* src = elems + (((uint64_t(offHi) << 32) | offLo) + (a - start));
-            const uint32_t n = b - a;
-            uint32_t cnt = 0;
-            for (uint32_t i = 0; i < n; i += U * WAVE) {
-                // straight-line loads: positions past the partner's end re-read its last element
-                uint32_t x[U];
-                bool v[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t k = i + uint32_t(u) * WAVE + lane;
-                    v[u] = k < n;
-                    x[u] = src[v[u] ? k : n - 1];
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) cnt += uint32_t(__popcll(__ballot(probeFn(x[u]) & v[u])));
-            }
+            return elems + ((uint64_t(offHi) << 32) | offLo);
+        };
+        // partner r's count for its elements [a, b) of its run [st, en)
+        auto finish = [&](uint32_t r, uint32_t st, uint32_t en, uint32_t a, uint32_t b, uint32_t cnt) {
             if (lane == 0) {
-                if (a == start && b == end) {
+                if (a == st && b == en) {
                     uint32_t* o = out + uint32_t(__builtin_amdgcn_readlane(int(R.w), int(r)));
                     if (ADD) atomicAdd(o, cnt); else *o = cnt;
                 } else {
-                    const int slot = a != start ? 0 : 1;
+                    const int slot = a != st ? 0 : 1;
                     cutSet[slot] = qb + r;
                     cutCnt[slot] = cnt;
                 }
             }
-            start = end;
+        };
+        uint32_t start = prevEnd;
+        for (uint32_t r = 0; r < nrec && start < we; r += 2) {
+            // partner r covers [start, endA), partner r + 1 [endA, endB); both
+            // have elements, and the wave's share begins inside partner r
+            const uint32_t endA = uint32_t(__builtin_amdgcn_readlane(int(R.z), int(r)));
+            const uint32_t aA = start > ws ? start : ws, bA = endA < we ? endA : we, nA = bA - aA;
+            const uint32_t* srcA = elemsOf(r) + (aA - start);
+            const bool hasB = r + 1 < nrec && endA < we;
+            uint32_t endB = endA, bB = endA, nB = 0;
+            const uint32_t* srcB = srcA;
+            if (hasB) {
+                endB = uint32_t(__builtin_amdgcn_readlane(int(R.z), int(r + 1)));
+                bB = endB < we ? endB : we;
+                nB = bB - endA;
+                srcB = elemsOf(r + 1);
+            }
+            uint32_t cntA = 0, cntB = 0;
+            const uint32_t n = nA > nB ? nA : nB;
+            for (uint32_t i = 0; i < n; i += U * WAVE) {
+                // straight-line loads: positions past a partner's end re-read
+                // its last element (partner B's first when it has none)
+                uint32_t xa[U], xb[U];
+                bool va[U], vb[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t k = i + uint32_t(u) * WAVE + lane;
+                    va[u] = k < nA;
+                    vb[u] = k < nB;
+                    xa[u] = srcA[va[u] ? k : nA - 1];
+                    xb[u] = srcB[vb[u] ? k : (nB ? nB - 1 : 0u)];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    cntA += uint32_t(__popcll(__ballot(probeFn(xa[u]) & va[u])));
+                    cntB += uint32_t(__popcll(__ballot(probeFn(xb[u]) & vb[u])));
+                }
+            }
+            finish(r, start, endA, aA, bA, cntA);
+            if (hasB) finish(r + 1, endA, endB, endA, bB, cntB);
+            start = hasB ? endB : endA;
         }
         prevEnd = uint32_t(__builtin_amdgcn_readlane(int(R.z), int(nrec - 1)));
         qb += nrec;
