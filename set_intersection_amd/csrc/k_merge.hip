@@ -55,6 +55,18 @@ using namespace si_dev;
 // an inclusive prefix ends the walk) and copies the slot to list[prefix ...).  The last segment writes
 // the total to out; the last workgroup to finish clears state and resets
 // done[0] and done[1].
+// Start of segment s when the nseg segment lengths rise linearly from
+// (1 - e) to (1 + e) times the mean, e = ramp / 256: n s ((1 - e)(nseg - 1) +
+// e (s - 1)) / (nseg (nseg - 1)), split so no product passes 64 bits
+// (nseg < 2^13, ramp < 256).  Monotone, 0 at s = 0 and n at s = nseg.
+__host__ __device__ inline uint64_t ramp_start(uint64_t n, uint32_t nseg, uint32_t ramp, uint32_t s) {
+    if (nseg < 2 || s == 0) return s == 0 ? 0 : n;
+    if (s >= nseg) return n;
+    const uint64_t q = uint64_t(s) * (uint64_t(256 - ramp) * (nseg - 1) + uint64_t(ramp) * (s - 1));
+    const uint64_t den = uint64_t(256) * nseg * (nseg - 1);
+    return (n / den) * q + ((n % den) * q) / den;
+}
+
 template <int NT, int VT, int WN, bool MAT, bool SEL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sisi_stream(const uint32_t* __restrict__ base, uint64_t oA, uint64_t nA,
                                                     uint64_t oB, uint64_t nB, uint64_t seg_len, uint32_t nseg,
@@ -62,7 +74,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                     uint64_t* __restrict__ out, uint32_t* __restrict__ out32,
                                                     uint32_t* __restrict__ stage, uint64_t stage_cap,
                                                     uint32_t* __restrict__ list, uint64_t* __restrict__ state,
-                                                    uint32_t prio) {
+                                                    uint32_t prio, uint32_t ramp) {
     __shared__ StreamSmem<NT, VT, WN, MAT ? 1024 : 1> sm;
     __shared__ uint32_t shTicket;
     __shared__ uint64_t shPrefix;
@@ -73,8 +85,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         s = shTicket;
     }
     const uint64_t n = nA + nB;
-    const uint64_t dBeg = uint64_t(s) * seg_len;
-    const uint64_t dEnd = (dBeg + seg_len < n) ? dBeg + seg_len : n;
+    uint64_t dBeg = uint64_t(s) * seg_len;
+    uint64_t dEnd = (dBeg + seg_len < n) ? dBeg + seg_len : n;
+    if (ramp) {  // segment lengths rising linearly with the ticket
+        dBeg = ramp_start(n, nseg, ramp, s);
+        dEnd = ramp_start(n, nseg, ramp, s + 1);
+    }
 
     if (threadIdx.x < WAVE) {
         const uint64_t dd = (lane_id() < 32) ? dBeg : dEnd;
@@ -228,13 +244,14 @@ struct SisiLaunch {
     uint64_t cap;
     uint32_t* list;
     uint64_t* state;
+    uint32_t ramp = 0;  // materialisation: segment length ramp (ramp_start)
 };
 
 template <int NT, int VT, int WN, bool SEL, bool MAT>
 static void launch_stream(const SisiLaunch& L) {
     hipLaunchKernelGGL((k_sisi_stream<NT, VT, WN, MAT, SEL>), dim3(L.nseg), dim3(NT), 0, L.ctx->stream, L.base, L.oA,
                        L.nA, L.oB, L.nB, L.seg_len, L.nseg, L.partials, L.done, L.out, L.out32, L.stage, L.cap,
-                       L.list, L.state, uint32_t(L.ctx->sisi_prio));
+                       L.list, L.state, uint32_t(L.ctx->sisi_prio), MAT ? L.ramp : 0u);
 }
 
 template <bool MAT>
@@ -303,8 +320,12 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
     uint32_t nseg;
     uint64_t seg_len;
     sisi_geometry(ctx, nA + nB, nullptr, kShapes[shape], nseg, seg_len);
-    // a segment of seg_len merged elements holds at most seg_len / 2 matches
-    const uint64_t cap = ((seg_len / 2 + 2) + 63) & ~uint64_t(63);
+    // a segment of L merged elements holds at most L / 2 matches; with the
+    // ramp the last segment is the longest
+    const uint32_t ramp = ctx->sisi_ramp > 0 && ctx->sisi_ramp < 128 && nseg >= 2 ? uint32_t(ctx->sisi_ramp) : 0u;
+    const uint64_t maxLen = ramp ? ramp_start(nA + nB, nseg, ramp, nseg) - ramp_start(nA + nB, nseg, ramp, nseg - 1)
+                                 : seg_len;
+    const uint64_t cap = (((maxLen > seg_len ? maxLen : seg_len) / 2 + 2) + 63) & ~uint64_t(63);
     void* ws = nullptr;
     SI_TRY(small_workspace(ctx, 256, &ws));
     SI_TRY(ctx->ws_stage.ensure(cap * nseg * sizeof(uint32_t)));
@@ -314,7 +335,7 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
         SI_HIP(hipMemsetAsync(ctx->ws_lookback.ptr, 0, ctx->ws_lookback.bytes, ctx->stream));
     }
     SisiLaunch L{ctx, nseg, ds->d_elems, oA, nA, oB, nB, seg_len, nullptr, static_cast<uint32_t*>(ws), d_count,
-                 nullptr, ctx->ws_stage.as<uint32_t>(), cap, d_out, ctx->ws_lookback.as<uint64_t>()};
+                 nullptr, ctx->ws_stage.as<uint32_t>(), cap, d_out, ctx->ws_lookback.as<uint64_t>(), ramp};
     KernelScope ks(ctx, "ip_sisi_materialize");
     launch_shape<true>(shape, L);
     SI_HIP(hipGetLastError());

@@ -84,6 +84,7 @@ struct si_context {
     bool ws_small_zeroed = false;
     int sisi_shape = 0;  // tile shape of the SISI merge kernel used by materialisation
     int sisi_prio = 1;   // SISI merge kernel rotates wave priority by tile (SI_SISI_PRIO=0 turns it off)
+    int sisi_ramp = 4;   // materialising segments grow by 2 ramp/256 of the mean from first to last (SI_SISI_RAMP)
     // si_pair_count_noalloc: this device's copy of the file-scope scratch
     // array, and whether the current call must take its workspace from it
     void* noalloc_ws = nullptr;

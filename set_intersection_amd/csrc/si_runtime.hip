@@ -221,6 +221,7 @@ int si_open(int device, si_context** out) {
     // tile shape of the SISI merge kernel for materialisation (tuning sweeps)
     if (const char* e = std::getenv("SI_SISI_SHAPE")) ctx->sisi_shape = std::atoi(e);
     if (const char* e = std::getenv("SI_SISI_PRIO")) ctx->sisi_prio = std::atoi(e);
+    if (const char* e = std::getenv("SI_SISI_RAMP")) ctx->sisi_ramp = std::atoi(e);
     *out = ctx;
     return SI_OK;
 }

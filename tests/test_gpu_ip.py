@@ -326,6 +326,28 @@ def test_priority_rotation(prio, monkeypatch):
         c.close()
 
 
+@pytest.mark.parametrize("ramp", [0, 4, 100])
+def test_segment_ramp(ramp, monkeypatch):
+    """Materialising segments whose lengths rise linearly with their ticket
+    (SI_SISI_RAMP / 256 of the mean either side; 4 by default, 100 a steep
+    ramp): the lists are exact for every tile shape's staging capacity."""
+    monkeypatch.setenv("SI_SISI_RAMP", str(ramp))
+    c = si.Context(0)
+    try:
+        pairs = list(_adversarial_pairs())
+        pairs.append((datasets.uniform_sorted(3_000_000, 1 << 26, 7), datasets.uniform_sorted(2_500_000, 1 << 26, 8)))
+        pairs.append((np.arange(0, 4_000_000, 2, dtype=np.uint32), np.arange(0, 4_000_000, 2, dtype=np.uint32)))
+        for a, b in pairs:
+            coll = si.Collection.from_sets([a, b])
+            ds = c.upload(coll)
+            exp = oracle.materialize(coll.set(0), coll.set(1))
+            assert np.array_equal(ds.pair_materialize(0, 1), exp), (ramp, a.size, b.size)
+            assert np.array_equal(ds.pair_materialize(1, 0), exp), (ramp, a.size, b.size)
+            ds.free()
+    finally:
+        c.close()
+
+
 def test_sisi_full_size():
     """The headline pair at full size: |A| = |B| = 2^28 from [0, 2^32).
     Exact against the oracle's std::set_intersection; the materialised list is
