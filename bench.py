@@ -77,12 +77,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(name):
-    """HBM bytes per launch recorded from rocprofv3 --pmc runs (or None)."""
+def read_traffic(name, label=None):
+    """HBM bytes per launch recorded from rocprofv3 --pmc runs (or None); a
+    summary that covers several kernels labels each line with the kernel."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    key = "traffic_bytes_per_launch" + (re.escape(f"[{label}]") if label else "")
+    m = re.search(key + r":\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -338,7 +340,9 @@ def bench_misi(args, si, datasets, cpu_rows=4096):
     pb = probe_bytes(coll.sizes)
     rec["roofline"] = {"bound": "hbm", "achieved": round(pb / (best_ms / 1e3) / 1e9, 2), "peak": HBM_PEAK_GBPS,
                        "unit": "GB/s", "frac": round(pb / (best_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
-                       "traffic": None, "kernel": "batched dispatcher (k_rows, k_merge_pairs), whole step",
+                       "traffic": read_traffic("k_rows_misi_16384_zipf_counters.md"),
+                       "traffic_source": "k_rows FETCH_SIZE + WRITE_SIZE per launch, default tuning",
+                       "kernel": "batched dispatcher (k_rows, k_merge_pairs), whole step",
                        "algorithmic_bytes": pb,
                        "basis": "4 B per partner element probed, sum over pairs of 4 min(|A|,|B|)",
                        "note": "a set's elements are probed once per larger partner, and these re-reads are "
@@ -390,8 +394,9 @@ def bench_mm(args, si, datasets):
     def mfma_roofline(name, peak):
         tops = ops_exec / (res[name]["kernel_ms"] / 1e3) / 1e12
         return {"bound": "mfma", "achieved": round(tops, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(tops / peak, 4), "traffic": None, "kernel": f"k_mm_bits<{name}>",
-                "ops_executed": ops_exec}
+                "frac": round(tops / peak, 4),
+                "traffic": read_traffic("k_popcount_pairs_8192x2^16_pmc.md", f"k_mm_bits<{name}>"),
+                "kernel": f"k_mm_bits<{name}>", "ops_executed": ops_exec}
 
     rec = base_record(args, 2.0 * k * k * Kp / (res["i8"]["ms"] / 1e3) / 1e12, "TOP/s (dense-equivalent)",
                       res["i8"]["ms"], {"workload": f"bitmap GEMM, {k} sets over 2^16, density U[1%,10%], "
@@ -467,7 +472,9 @@ def bench_edges(args, si, datasets):
     # one's LDS table: 4 min(|N(u)|, |N(v)|) bytes per edge
     pb = 4.0 * float(np.minimum(coll.sizes[u], coll.sizes[v]).sum(dtype=np.float64))
     rec["roofline"] = {"bound": "hbm", "achieved": round(pb / (ms / 1e3) / 1e9, 2), "peak": HBM_PEAK_GBPS,
-                       "unit": "GB/s", "frac": round(pb / (ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4), "traffic": None,
+                       "unit": "GB/s", "frac": round(pb / (ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
+                       "traffic": read_traffic("k_edges_wg_rmat20_counters.md"),
+                       "traffic_source": "k_edges_wg + k_edges_wave FETCH_SIZE + WRITE_SIZE per step, default tuning",
                        "kernel": "edge dispatcher (k_edges_wg, k_edges_wave), whole step", "algorithmic_bytes": pb,
                        "basis": "4 B per partner element probed, sum over edges of 4 min(|N(u)|,|N(v)|)"}
     rec["exact"] = exact
