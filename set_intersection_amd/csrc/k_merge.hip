@@ -57,8 +57,9 @@ using namespace si_dev;
 // done[0] and done[1].
 // Start of segment s when the nseg segment lengths rise linearly from
 // (1 - e) to (1 + e) times the mean, e = ramp / 256: n s ((1 - e)(nseg - 1) +
-// e (s - 1)) / (nseg (nseg - 1)), split so no product passes 64 bits
-// (nseg < 2^13, ramp < 256).  Monotone, 0 at s = 0 and n at s = nseg.
+// e (s - 1)) / (nseg (nseg - 1)), split so no product passes 64 bits:
+// q and den are below 2^8 nseg^2, so (n % den) q < 2^16 nseg^4 needs
+// nseg < 2^12 (ramp < 256).  Monotone, 0 at s = 0 and n at s = nseg.
 __host__ __device__ inline uint64_t ramp_start(uint64_t n, uint32_t nseg, uint32_t ramp, uint32_t s) {
     if (nseg < 2 || s == 0) return s == 0 ? 0 : n;
     if (s >= nseg) return n;
@@ -322,7 +323,8 @@ int ip_pair_materialize(si_dataset* ds, uint32_t a, uint32_t b, uint32_t* d_out,
     sisi_geometry(ctx, nA + nB, nullptr, kShapes[shape], nseg, seg_len);
     // a segment of L merged elements holds at most L / 2 matches; with the
     // ramp the last segment is the longest
-    const uint32_t ramp = ctx->sisi_ramp > 0 && ctx->sisi_ramp < 128 && nseg >= 2 ? uint32_t(ctx->sisi_ramp) : 0u;
+    const uint32_t ramp =
+        ctx->sisi_ramp > 0 && ctx->sisi_ramp < 128 && nseg >= 2 && nseg < 4096 ? uint32_t(ctx->sisi_ramp) : 0u;
     const uint64_t maxLen = ramp ? ramp_start(nA + nB, nseg, ramp, nseg) - ramp_start(nA + nB, nseg, ramp, nseg - 1)
                                  : seg_len;
     const uint64_t cap = (((maxLen > seg_len ? maxLen : seg_len) / 2 + 2) + 63) & ~uint64_t(63);
